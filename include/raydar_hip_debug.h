@@ -1,0 +1,63 @@
+/*
+ * raydar_hip_debug.h — per-stage parity hooks of the HIP backend.
+ *
+ * These entry points have no counterpart in the reference: raydar's renderer
+ * exposes whole frames only (src/renderer/mod.rs:25-35).  They run single
+ * stages of the GPU hot path on caller-supplied inputs so the test-suite can
+ * compare each stage with the CPU oracle on identical numbers:
+ *   camera rays      cpu.rs:234-251
+ *   closest hit      cpu.rs:344-394 (BVH or linear scan)
+ *   scatter          cpu.rs:262-333 with explicit draw words
+ *   world sample     world.rs:17-34
+ *   Philox-4x32-10   the counter RNG that replaces thread_rng
+ * Device hooks need a GPU; the rd_debug_host_* hooks run the BVH builder and
+ * its traversal rule on the host (no GPU) for the builder's CPU tests.
+ */
+#ifndef RAYDAR_HIP_DEBUG_H
+#define RAYDAR_HIP_DEBUG_H
+
+#include "raydar_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Closest hit of n rays (n x 6 floats: origin, direction) on the GPU.
+ * ids: object index or -1; ts: hit distance (0 on miss); normals: n x 3
+ * closest_hit normal after the front-face flip; front: front-face flag.
+ * brute != 0 scans every object instead of traversing the BVH. */
+rd_status rd_debug_trace(const rd_scene *scene, int leaf_size, int brute, const float *rays,
+                         uint32_t n, int32_t *ids, float *ts, float *normals, uint8_t *front);
+
+/* Scatter of n hits on the GPU.  in: n x 16 floats = ray origin(3),
+ * direction(3), t, object id (uint32 bits), attenuation(3), light(3), 2 unused;
+ * bits: n x 12 draw words (slot order of cpu.rs:262-314).  out: n x 16 floats
+ * = new origin(3), new direction(3), attenuation(3), light(3), transmission
+ * flag, normal(3). */
+rd_status rd_debug_scatter(const rd_scene *scene, const float *in, const uint32_t *bits,
+                           uint32_t n, float *out);
+
+/* Camera rays (n x 6) for the given pixel indices (y * W + x). */
+rd_status rd_debug_camera_rays(const rd_scene *scene, const uint32_t *pixels, uint32_t n,
+                               float *out);
+
+/* World colour (n x 3) for n directions. */
+rd_status rd_debug_world(const rd_scene *scene, const float *dirs, uint32_t n, float *out);
+
+/* Philox-4x32-10 on the GPU: in n x 6 words (counter[4], key[2]), out n x 4. */
+rd_status rd_debug_philox(const uint32_t *in, uint32_t n, uint32_t *out);
+
+/* Host: build the BVH and report node count, depth and whether its
+ * structural invariants hold (message in err when they do not). */
+rd_status rd_debug_host_bvh_info(const rd_scene *scene, int leaf_size, uint32_t *n_nodes,
+                                 uint32_t *depth, char *err, size_t err_capacity);
+
+/* Host: closest hit through the BVH with the kernels' culling rule. */
+rd_status rd_debug_host_trace(const rd_scene *scene, int leaf_size, const float *rays,
+                              uint32_t n, int32_t *ids, float *ts);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RAYDAR_HIP_DEBUG_H */

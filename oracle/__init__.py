@@ -1,0 +1,131 @@
+"""ctypes access to the oracle library (oracle/oracle.cpp).
+
+TEST INFRASTRUCTURE ONLY: tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg use it as the checker or the timed CPU baseline.  The
+product package raydar_amd never imports it.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from raydar_amd import abi
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIBS = {}
+
+_P = np.ctypeslib.ndpointer
+
+
+def build():
+    """Compile liboracle.so / liboracle_fast.so with g++ (see Makefile)."""
+    import subprocess
+    subprocess.run(["make", "-s", "-C", _HERE, "-B", "all"], check=True)
+
+
+def lib(fast=False):
+    name = "liboracle_fast.so" if fast else "liboracle.so"
+    if name in _LIBS:
+        return _LIBS[name]
+    path = os.path.join(_HERE, name)
+    if not os.path.exists(path):
+        build()
+    L = C.CDLL(path)
+    sp = C.POINTER(abi.RdScene)
+    L.or_philox4x32_10.argtypes = [_P(np.uint32), _P(np.uint32), _P(np.uint32)]
+    L.or_bounce_bits.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, _P(np.uint32)]
+    L.or_camera_rays_f32.argtypes = [sp, _P(np.uint32), C.c_uint32, _P(np.float32)]
+    L.or_trace_f32.argtypes = [sp, _P(np.float32), C.c_uint32, _P(np.int32), _P(np.float32),
+                               _P(np.float32), _P(np.uint8)]
+    L.or_scatter_f32.argtypes = [sp, _P(np.float32), _P(np.uint32), C.c_uint32, _P(np.float32)]
+    L.or_world_f32.argtypes = [sp, _P(np.float32), C.c_uint32, _P(np.float32)]
+    L.or_render_f32.argtypes = [sp, C.c_uint32, C.c_uint32, C.c_uint64, C.c_uint32, C.c_void_p,
+                                C.c_uint32, _P(np.float32), C.c_int]
+    L.or_render_f64.argtypes = [sp, C.c_uint32, C.c_uint32, C.c_uint64, C.c_uint32, C.c_void_p,
+                                C.c_uint32, _P(np.float64), C.c_int]
+    L.or_tonemap.argtypes = [_P(np.float32), C.c_uint64, C.c_uint32, _P(np.uint8)]
+    for f in ("or_philox4x32_10", "or_bounce_bits", "or_camera_rays_f32", "or_trace_f32",
+              "or_scatter_f32", "or_world_f32", "or_render_f32", "or_render_f64", "or_tonemap"):
+        getattr(L, f).restype = None
+    _LIBS[name] = L
+    return L
+
+
+def _c(a, dt):
+    return np.ascontiguousarray(a, dtype=dt)
+
+
+def philox(ctr, key):
+    out = np.zeros(4, np.uint32)
+    lib().or_philox4x32_10(_c(ctr, np.uint32), _c(key, np.uint32), out)
+    return out
+
+
+def bounce_bits(seed, pixel, sample, bounce):
+    out = np.zeros(12, np.uint32)
+    lib().or_bounce_bits(seed, pixel, sample, bounce, out)
+    return out
+
+
+def camera_rays(scene, pixels):
+    pixels = _c(pixels, np.uint32)
+    out = np.zeros((len(pixels), 6), np.float32)
+    s = scene.as_rd()
+    lib().or_camera_rays_f32(C.byref(s), pixels, len(pixels), out)
+    return out
+
+
+def trace(scene, rays):
+    rays = _c(rays, np.float32).reshape(-1, 6)
+    n = len(rays)
+    ids = np.zeros(n, np.int32)
+    ts = np.zeros(n, np.float32)
+    normals = np.zeros((n, 3), np.float32)
+    front = np.zeros(n, np.uint8)
+    s = scene.as_rd()
+    lib().or_trace_f32(C.byref(s), rays, n, ids, ts, normals, front)
+    return ids, ts, normals, front
+
+
+def scatter(scene, packed, bits):
+    """packed: n x 16 floats (see oracle.cpp or_scatter_f32); bits: n x 12."""
+    packed = _c(packed, np.float32).reshape(-1, 16)
+    bits = _c(bits, np.uint32).reshape(-1, 12)
+    out = np.zeros_like(packed)
+    s = scene.as_rd()
+    lib().or_scatter_f32(C.byref(s), packed, bits, len(packed), out)
+    return out
+
+
+def world(scene, dirs):
+    dirs = _c(dirs, np.float32).reshape(-1, 3)
+    out = np.zeros_like(dirs)
+    s = scene.as_rd()
+    lib().or_world_f32(C.byref(s), dirs, len(dirs), out)
+    return out
+
+
+def render(scene, spp, max_bounces, seed, sample_base=0, pixels=None, fp64=False, threads=None,
+           fast=False):
+    """Accumulator (n x 4) of `spp` samples for `pixels` (all when None)."""
+    if threads is None:
+        threads = os.cpu_count() or 1
+    n = scene.width * scene.height if pixels is None else len(pixels)
+    acc = np.zeros((n, 4), np.float64 if fp64 else np.float32)
+    pix = None
+    if pixels is not None:
+        pix = _c(pixels, np.uint32)
+    s = scene.as_rd()
+    L = lib(fast)
+    fn = L.or_render_f64 if fp64 else L.or_render_f32
+    fn(C.byref(s), spp, max_bounces, seed, sample_base,
+       None if pix is None else pix.ctypes.data_as(C.c_void_p), 0 if pix is None else len(pix),
+       acc, threads)
+    return acc
+
+
+def tonemap(acc, sample_count):
+    acc = _c(acc, np.float32)
+    out = np.zeros(acc.size, np.uint8)
+    lib().or_tonemap(acc.reshape(-1), acc.size // 4, sample_count, out)
+    return out

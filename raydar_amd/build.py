@@ -1,0 +1,85 @@
+"""Build libraydar_hip.so (HIP kernels for gfx950 + host C++) in-tree.
+
+    python -m raydar_amd.build
+
+HIP sources are compiled with hipcc for gfx950 only, with
+  -ffp-contract=off          no a*b+c fusion: the kernels must reproduce the
+                             reference's FP32 operation sequence
+  -target-feature -packed-fp32-ops
+                             keep the compiler from emitting packed FP32 ops
+Host sources (.cpp) are compiled with g++ -ffp-contract=off for the same
+reason (they pre-compute FP32 values the kernels consume).
+"""
+import os
+import shutil
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+BUILD = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "libraydar_hip.so")
+CLI = os.path.join(HERE, "raydar_hip")
+
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+HIPCC = os.path.join(ROCM, "bin", "hipcc")
+ARCH = "gfx950"
+
+HIP_FLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wall",
+]
+CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wextra",
+             "-Wno-unknown-pragmas"]
+
+HIP_SOURCES = ["kernels.hip", "engine.hip", "api.hip", "batch.hip"]
+CXX_SOURCES = ["bvh.cpp", "rscn.cpp"]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    out = "\n".join(l for l in (r.stdout + r.stderr).splitlines()
+                    if "packed-fp32-ops" not in l and "not a recognized feature" not in l)
+    if r.returncode != 0:
+        raise RuntimeError(f"command failed ({r.returncode}): {' '.join(cmd)}\n{out}")
+    return out
+
+
+def build(verbose=False):
+    os.makedirs(BUILD, exist_ok=True)
+    jobs = []
+    for s in HIP_SOURCES:
+        src = os.path.join(CSRC, s)
+        if os.path.exists(src):
+            jobs.append([HIPCC, *HIP_FLAGS, "-c", src, "-o", os.path.join(BUILD, s + ".o")])
+    for s in CXX_SOURCES:
+        jobs.append(["g++", *CXX_FLAGS, "-c", os.path.join(CSRC, s), "-o",
+                     os.path.join(BUILD, s + ".o")])
+    with ThreadPoolExecutor(max_workers=min(8, len(jobs))) as ex:
+        for out in ex.map(_run, jobs):
+            if verbose and out.strip():
+                print(out)
+    objs = [j[-1] for j in jobs]
+    tmp = LIB + ".tmp"
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
+    os.replace(tmp, LIB)
+    cli_src = os.path.join(CSRC, "raydar_cli.cpp")
+    if os.path.exists(cli_src):
+        _run(["g++", *CXX_FLAGS, "-O2", "-o", CLI + ".tmp", cli_src, "-L" + HERE,
+              "-lraydar_hip", "-lz", "-Wl,-rpath,$ORIGIN"])
+        os.replace(CLI + ".tmp", CLI)
+    return LIB
+
+
+def clean():
+    shutil.rmtree(BUILD, ignore_errors=True)
+    for p in (LIB, CLI):
+        if os.path.exists(p):
+            os.remove(p)
+
+
+if __name__ == "__main__":
+    if "--clean" in sys.argv:
+        clean()
+    print(build(verbose="-v" in sys.argv))

@@ -1,0 +1,648 @@
+// C ABI of the per-scene HIP renderer (include/raydar_hip.h) and the parity
+// hooks (include/raydar_hip_debug.h).
+//
+// rd_renderer mirrors CpuRenderer (src/renderer/cpu.rs:110-116): a config,
+// a sample count, a profiler, and an accumulator that persists across
+// render_sample calls — here a device-resident RGBA32F buffer next to the
+// scene's device image (BVH, primitives, materials) snapshotted at new_frame.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/raydar_hip.h"
+#include "../../include/raydar_hip_debug.h"
+#include "bvh.hpp"
+#include "engine.hpp"
+#include "kernels.hpp"
+
+namespace rdh {
+
+namespace {
+std::mutex g_err_mu;
+std::string g_err;
+
+// timing.rs:66-127 Timer: optional start, optional duration.
+struct Timer {
+  bool started = false, has_duration = false;
+  std::chrono::steady_clock::time_point t0;
+  int64_t ns = -1;
+  void start() {
+    started = true;
+    t0 = std::chrono::steady_clock::now();
+  }
+  void start_if_not_started() {
+    if (!started) start();
+  }
+  void end() {
+    if (started) {
+      has_duration = true;
+      ns = std::chrono::duration_cast<std::chrono::nanoseconds>(std::chrono::steady_clock::now() - t0)
+               .count();
+    }
+  }
+  void end_if_not_ended() {
+    if (!has_duration) end();
+  }
+};
+}  // namespace
+
+void set_global_error(const std::string &msg) {
+  std::lock_guard<std::mutex> lk(g_err_mu);
+  g_err = msg;
+}
+
+}  // namespace rdh
+
+struct rd_renderer {
+  rd_config cfg{1024, 12};
+  uint32_t sample_count = 0;
+  uint32_t samples_per_pass = 1;
+  uint64_t seed = 0;
+  std::string last_error;
+  rdh::Engine eng;
+  rdh::DeviceScene scene;
+  rdh::DevJob job_h{};
+  rdh::DevJob *job_d = nullptr;
+  float *acc = nullptr;
+  uchar4 *rgba_d = nullptr;
+  uint32_t W = 0, H = 0;
+  bool has_frame = false;
+  rdh::Timer frame_timer, sample_timer, prepare_timer, render_timer;
+
+  rd_status fail(rd_status st, const std::string &msg) {
+    last_error = msg;
+    return st;
+  }
+};
+
+namespace {
+
+rd_status device_fail(rd_renderer *r, const std::string &msg) {
+  r->last_error = msg;
+  return msg.find("out of memory") != std::string::npos ? RD_ERR_OUT_OF_MEMORY : RD_ERR_DEVICE;
+}
+
+bool valid_scene(const rd_scene *s, std::string *err) {
+  if (!s) { *err = "scene is NULL"; return false; }
+  if (s->camera.resolution_x == 0 || s->camera.resolution_y == 0) {
+    *err = "zero-size image";
+    return false;
+  }
+  if ((uint64_t)s->camera.resolution_x * s->camera.resolution_y > (1ull << 28)) {
+    *err = "image too large (more than 2^28 pixels)";
+    return false;
+  }
+  if (s->world.kind == RD_WORLD_TRANSPARENT) {
+    // World::sample -> todo!("transparent world support") (world.rs:31)
+    *err = "transparent world is not supported (raydar: todo!(\"transparent world support\"))";
+    return false;
+  }
+  if (s->world.kind > RD_WORLD_TRANSPARENT) { *err = "unknown world kind"; return false; }
+  return true;
+}
+
+// Snapshot a scene into device memory and (re)allocate the accumulator.
+rd_status snapshot(rd_renderer *r, const rd_scene *s, bool zero_acc) {
+  std::string err;
+  if (!valid_scene(s, &err)) return r->fail(s && s->world.kind == RD_WORLD_TRANSPARENT
+                                                ? RD_ERR_UNSUPPORTED
+                                                : RD_ERR_INVALID_ARGUMENT,
+                                            err);
+  rdh::BuiltScene built;
+  if (!rdh::build_scene(*s, r->eng.opt.leaf, &built, &err)) return r->fail(RD_ERR_INVALID_ARGUMENT, err);
+  if (!rdh::check_bvh(built, &err)) return r->fail(RD_ERR_INVALID_ARGUMENT, "BVH build: " + err);
+  if (!r->eng.upload_scene(built, &r->scene, &err)) return device_fail(r, err);
+  const uint32_t W = s->camera.resolution_x, H = s->camera.resolution_y;
+  if (W != r->W || H != r->H || !r->acc) {
+    if (r->acc) (void)hipFree(r->acc);
+    if (r->rgba_d) (void)hipFree(r->rgba_d);
+    r->acc = nullptr;
+    r->rgba_d = nullptr;
+    if (!r->eng.check(hipMalloc(&r->acc, 16ull * W * H), "hipMalloc(accumulator)", &err) ||
+        !r->eng.check(hipMalloc(&r->rgba_d, 4ull * W * H), "hipMalloc(image)", &err))
+      return device_fail(r, err);
+    r->W = W;
+    r->H = H;
+    zero_acc = true;
+  }
+  if (zero_acc && !r->eng.check(rdh::launch_zero_acc(r->acc, 4ull * W * H, r->eng.stream),
+                                "zero accumulator", &err))
+    return device_fail(r, err);
+  rdh::fill_job(*s, r->scene, r->acc, r->cfg.max_bounces, r->seed, &r->job_h);
+  if (!r->job_d && !r->eng.check(hipMalloc(&r->job_d, sizeof(rdh::DevJob)), "hipMalloc(job)", &err))
+    return device_fail(r, err);
+  if (!r->eng.check(hipMemcpyAsync(r->job_d, &r->job_h, sizeof(rdh::DevJob),
+                                   hipMemcpyHostToDevice, r->eng.stream),
+                    "upload job", &err) ||
+      !r->eng.check(hipStreamSynchronize(r->eng.stream), "upload job", &err))
+    return device_fail(r, err);
+  r->has_frame = true;
+  return RD_OK;
+}
+
+// Job fields that can change between passes without a new snapshot.
+rd_status refresh_job(rd_renderer *r) {
+  std::string err;
+  r->job_h.max_bounces = r->cfg.max_bounces;
+  r->job_h.seed = r->seed;
+  if (!r->eng.check(hipMemcpyAsync(r->job_d, &r->job_h, sizeof(rdh::DevJob),
+                                   hipMemcpyHostToDevice, r->eng.stream),
+                    "upload job", &err))
+    return device_fail(r, err);
+  return RD_OK;
+}
+
+// Renders `k` more samples (cpu.rs:193-219 repeated k times).
+rd_status render_samples(rd_renderer *r, uint32_t k) {
+  r->prepare_timer.end_if_not_ended();
+  r->render_timer.start_if_not_started();
+  r->sample_timer.start();
+  rd_status st = refresh_job(r);
+  if (st != RD_OK) return st;
+  std::string err;
+  const uint64_t npix = (uint64_t)r->W * r->H;
+  std::vector<rdh::DevJob> host_jobs{r->job_h};
+  uint32_t done = 0;
+  while (done < k) {
+    uint64_t fit = r->eng.opt.l_capacity / npix;
+    if (fit == 0) return r->fail(RD_ERR_INVALID_ARGUMENT, "image larger than the radiance arena");
+    uint32_t spp = (uint32_t)std::min<uint64_t>({(uint64_t)(k - done), fit, 0xffffffffull / npix});
+    rdh::DevTicket t{};
+    t.job = 0;
+    t.sample_base = r->sample_count + done;
+    t.spp = spp;
+    t.paths = (uint32_t)(spp * npix);
+    if (!r->eng.run_tickets(r->job_d, {t}, host_jobs, &err)) return device_fail(r, err);
+    done += spp;
+  }
+  if (!r->eng.check(hipStreamSynchronize(r->eng.stream), "render", &err)) return device_fail(r, err);
+  r->sample_count += k;
+  if (r->sample_count == r->cfg.max_sample_count) {
+    r->render_timer.end();
+    r->frame_timer.end();
+  }
+  r->sample_timer.end();
+  return RD_OK;
+}
+
+// print_frame_buffer (cpu.rs:221-230) on the device, then copy out.
+rd_status tonemap_out(rd_renderer *r, uint8_t *rgba, size_t cap) {
+  std::string err;
+  const uint64_t npix = (uint64_t)r->W * r->H;
+  const int grid = (int)std::min<uint64_t>((npix + 255) / 256, 8192);
+  if (!r->eng.check(rdh::launch_tonemap(r->job_d, (uint32_t)npix, (float)r->sample_count,
+                                        r->rgba_d, r->eng.counters, r->eng.opt.count != 0, grid,
+                                        r->eng.stream),
+                    "tonemap", &err) ||
+      !r->eng.check(hipMemcpyAsync(rgba, r->rgba_d, 4 * npix, hipMemcpyDeviceToHost, r->eng.stream),
+                    "read image", &err) ||
+      !r->eng.check(hipStreamSynchronize(r->eng.stream), "read image", &err))
+    return device_fail(r, err);
+  (void)cap;
+  return RD_OK;
+}
+
+bool option_value(const rd_renderer *r, const std::string &k, int64_t *v) {
+  const rdh::EngineOptions &o = r->eng.opt;
+  if (k == "block") *v = o.block;
+  else if (k == "queue") *v = o.queue;
+  else if (k == "leaf") *v = o.leaf;
+  else if (k == "grid") *v = o.grid;
+  else if (k == "traversal") *v = o.brute_force;
+  else if (k == "count") *v = o.count;
+  else if (k == "kernel_timing") *v = o.kernel_timing;
+  else if (k == "l_capacity") *v = (int64_t)o.l_capacity;
+  else if (k == "sync_every") *v = o.sync_every;
+  else if (k == "samples_per_pass") *v = r->samples_per_pass;
+  else if (k == "bvh_depth") *v = r->scene.depth;
+  else if (k == "bvh_nodes") *v = r->scene.n_nodes;
+  else if (k == "iterations") *v = (int64_t)r->eng.iterations;
+  else if (k == "num_cus") *v = r->eng.num_cus;
+  else return false;
+  return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+rd_status rd_config_default(rd_config *out) {
+  if (!out) return RD_ERR_INVALID_ARGUMENT;
+  out->max_sample_count = 1024;  // renderer/mod.rs:16-23
+  out->max_bounces = 12;
+  return RD_OK;
+}
+
+rd_status rd_create(const rd_config *config, rd_renderer **out) {
+  if (!out) {
+    rdh::set_global_error("rd_create: out is NULL");
+    return RD_ERR_INVALID_ARGUMENT;
+  }
+  *out = nullptr;
+  rd_renderer *r = new (std::nothrow) rd_renderer();
+  if (!r) return RD_ERR_OUT_OF_MEMORY;
+  if (config) r->cfg = *config;
+  std::string err;
+  if (!r->eng.init(&err)) {
+    rdh::set_global_error(err);
+    delete r;
+    return RD_ERR_DEVICE;
+  }
+  *out = r;
+  return RD_OK;
+}
+
+void rd_destroy(rd_renderer *r) {
+  if (!r) return;
+  (void)hipDeviceSynchronize();
+  r->eng.free_scene(&r->scene);
+  if (r->acc) (void)hipFree(r->acc);
+  if (r->rgba_d) (void)hipFree(r->rgba_d);
+  if (r->job_d) (void)hipFree(r->job_d);
+  delete r;
+}
+
+rd_status rd_new_frame(rd_renderer *r, const rd_scene *scene) {
+  if (!r) return RD_ERR_INVALID_ARGUMENT;
+  r->frame_timer.start();    // cpu.rs:136-137
+  r->prepare_timer.start();
+  r->sample_count = 0;
+  return snapshot(r, scene, true);
+}
+
+rd_status rd_restart_frame(rd_renderer *r) {
+  if (!r) return RD_ERR_INVALID_ARGUMENT;
+  if (!r->has_frame) return r->fail(RD_ERR_INVALID_ARGUMENT, "restart_frame before new_frame");
+  r->frame_timer.start();
+  r->prepare_timer.start();
+  r->sample_count = 0;
+  std::string err;
+  if (!r->eng.check(rdh::launch_zero_acc(r->acc, 4ull * r->W * r->H, r->eng.stream),
+                    "zero accumulator", &err))
+    return device_fail(r, err);
+  return RD_OK;
+}
+
+rd_status rd_render_sample(rd_renderer *r, const rd_scene *scene, uint8_t *rgba, size_t cap,
+                           int *produced) {
+  if (!r || !produced) return RD_ERR_INVALID_ARGUMENT;
+  *produced = 0;
+  if (r->sample_count >= r->cfg.max_sample_count) return RD_OK;  // cpu.rs:143-145 -> None
+  if (!r->has_frame || (scene && (scene->camera.resolution_x != r->W ||
+                                  scene->camera.resolution_y != r->H))) {
+    // frame_buffer(): a fresh buffer when none exists or the size changed
+    // (cpu.rs:400-417); sample_count is kept, as there.
+    if (!scene) return r->fail(RD_ERR_INVALID_ARGUMENT, "render_sample before new_frame needs a scene");
+    rd_status st = snapshot(r, scene, true);
+    if (st != RD_OK) return st;
+  }
+  if (!rgba || cap < 4ull * r->W * r->H)
+    return r->fail(RD_ERR_BUFFER_TOO_SMALL, "image buffer smaller than W*H*4 bytes");
+  const uint32_t k = std::min(r->samples_per_pass, r->cfg.max_sample_count - r->sample_count);
+  rd_status st = render_samples(r, k);
+  if (st != RD_OK) return st;
+  st = tonemap_out(r, rgba, cap);
+  if (st != RD_OK) return st;
+  *produced = 1;
+  return RD_OK;
+}
+
+rd_status rd_render_frame(rd_renderer *r, const rd_scene *scene, uint8_t *rgba, size_t cap) {
+  if (!r) return RD_ERR_INVALID_ARGUMENT;
+  if (scene && (!rgba || cap < 4ull * scene->camera.resolution_x * scene->camera.resolution_y))
+    return r->fail(RD_ERR_BUFFER_TOO_SMALL, "image buffer smaller than W*H*4 bytes");
+  rd_status st = rd_new_frame(r, scene);
+  if (st != RD_OK) return st;
+  if (r->sample_count < r->cfg.max_sample_count) {
+    st = render_samples(r, r->cfg.max_sample_count - r->sample_count);
+    if (st != RD_OK) return st;
+  }
+  return tonemap_out(r, rgba, cap);
+}
+
+rd_status rd_profiler(const rd_renderer *r, rd_timings *out) {
+  if (!r || !out) return RD_ERR_INVALID_ARGUMENT;
+  out->frame_ns = r->frame_timer.ns;
+  out->sample_ns = r->sample_timer.ns;
+  out->prepare_ns = r->prepare_timer.ns;
+  out->render_ns = r->render_timer.ns;
+  return RD_OK;
+}
+
+uint32_t rd_sample_count(const rd_renderer *r) { return r ? r->sample_count : 0; }
+uint32_t rd_max_sample_count(const rd_renderer *r) { return r ? r->cfg.max_sample_count : 0; }
+uint32_t rd_max_bounces(const rd_renderer *r) { return r ? r->cfg.max_bounces : 0; }
+void rd_set_max_sample_count(rd_renderer *r, uint32_t count) {
+  if (r) r->cfg.max_sample_count = count;
+}
+void rd_set_max_bounces(rd_renderer *r, uint32_t bounces) {
+  if (r) r->cfg.max_bounces = bounces;
+}
+
+const char *rd_last_error(const rd_renderer *r) {
+  if (r) return r->last_error.c_str();
+  static thread_local std::string copy;
+  std::lock_guard<std::mutex> lk(rdh::g_err_mu);
+  copy = rdh::g_err;
+  return copy.c_str();
+}
+
+rd_status rd_set_samples_per_pass(rd_renderer *r, uint32_t samples) {
+  if (!r) return RD_ERR_INVALID_ARGUMENT;
+  if (samples == 0) return r->fail(RD_ERR_INVALID_ARGUMENT, "samples per pass must be >= 1");
+  r->samples_per_pass = samples;
+  return RD_OK;
+}
+
+void rd_set_seed(rd_renderer *r, uint64_t seed) {
+  if (r) r->seed = seed;
+}
+
+rd_status rd_read_accumulator(rd_renderer *r, float *out, size_t cap) {
+  if (!r || !out) return RD_ERR_INVALID_ARGUMENT;
+  if (!r->has_frame) return r->fail(RD_ERR_INVALID_ARGUMENT, "no frame");
+  const uint64_t n = 4ull * r->W * r->H;
+  if (cap < n) return r->fail(RD_ERR_BUFFER_TOO_SMALL, "accumulator buffer too small");
+  std::string err;
+  if (!r->eng.check(hipMemcpyAsync(out, r->acc, 4 * n, hipMemcpyDeviceToHost, r->eng.stream),
+                    "read accumulator", &err) ||
+      !r->eng.check(hipStreamSynchronize(r->eng.stream), "read accumulator", &err))
+    return device_fail(r, err);
+  return RD_OK;
+}
+
+rd_status rd_set_option(rd_renderer *r, const char *key, int64_t v) {
+  if (!r || !key) return RD_ERR_INVALID_ARGUMENT;
+  const std::string k(key);
+  rdh::EngineOptions &o = r->eng.opt;
+  if (k == "block") {
+    if (v != 64 && v != 128 && v != 256) return r->fail(RD_ERR_INVALID_ARGUMENT, "block must be 64, 128 or 256");
+    o.block = (int)v;
+  } else if (k == "queue") {
+    if (v < 1024 || v > (1ll << 28)) return r->fail(RD_ERR_INVALID_ARGUMENT, "queue must be in [1024, 2^28]");
+    o.queue = (uint32_t)v;
+  } else if (k == "leaf") {
+    if (v < 1 || v > rdh::kMaxLeafSize) return r->fail(RD_ERR_INVALID_ARGUMENT, "leaf must be in 1..8");
+    o.leaf = (int)v;
+  } else if (k == "grid") {
+    if (v < 0 || v > (1 << 20)) return r->fail(RD_ERR_INVALID_ARGUMENT, "grid out of range");
+    o.grid = (int)v;
+  } else if (k == "traversal") {
+    if (v != 0 && v != 1) return r->fail(RD_ERR_INVALID_ARGUMENT, "traversal must be 0 or 1");
+    o.brute_force = (int)v;
+  } else if (k == "count") {
+    o.count = v != 0;
+  } else if (k == "kernel_timing") {
+    o.kernel_timing = v != 0;
+  } else if (k == "l_capacity") {
+    if (v < (1 << 16) || v > (1ll << 32)) return r->fail(RD_ERR_INVALID_ARGUMENT, "l_capacity out of range");
+    o.l_capacity = (uint64_t)v;
+  } else if (k == "sync_every") {
+    if (v < 1 || v > 1024) return r->fail(RD_ERR_INVALID_ARGUMENT, "sync_every out of range");
+    o.sync_every = (int)v;
+  } else if (k == "samples_per_pass") {
+    return rd_set_samples_per_pass(r, (uint32_t)std::max<int64_t>(0, std::min<int64_t>(v, 0xffffffffll)));
+  } else {
+    return r->fail(RD_ERR_INVALID_ARGUMENT, "unknown option " + k);
+  }
+  return RD_OK;
+}
+
+rd_status rd_get_option(const rd_renderer *r, const char *key, int64_t *v) {
+  if (!r || !key || !v) return RD_ERR_INVALID_ARGUMENT;
+  return option_value(r, key, v) ? RD_OK : RD_ERR_INVALID_ARGUMENT;
+}
+
+rd_status rd_get_counters(const rd_renderer *r, rd_counters *out) {
+  if (!r || !out) return RD_ERR_INVALID_ARGUMENT;
+  rdh::DevCounters c;
+  if (hipMemcpy(&c, r->eng.counters, sizeof(c), hipMemcpyDeviceToHost) != hipSuccess)
+    return RD_ERR_DEVICE;
+  static_assert(sizeof(rd_counters) == sizeof(rdh::DevCounters), "counter layout");
+  std::memcpy(out, &c, sizeof(c));
+  return RD_OK;
+}
+
+rd_status rd_reset_counters(rd_renderer *r) {
+  if (!r) return RD_ERR_INVALID_ARGUMENT;
+  if (hipMemsetAsync(r->eng.counters, 0, sizeof(rdh::DevCounters), r->eng.stream) != hipSuccess ||
+      hipStreamSynchronize(r->eng.stream) != hipSuccess)
+    return RD_ERR_DEVICE;
+  for (int i = 0; i < 5; ++i) {
+    r->eng.kernel_ms[i] = 0.0;
+    r->eng.kernel_launches[i] = 0;
+  }
+  r->eng.iterations = 0;
+  return RD_OK;
+}
+
+rd_status rd_get_kernel_times(const rd_renderer *r, rd_kernel_times *out) {
+  if (!r || !out) return RD_ERR_INVALID_ARGUMENT;
+  for (int i = 0; i < 5; ++i) {
+    out->total_ms[i] = r->eng.kernel_ms[i];
+    out->launches[i] = r->eng.kernel_launches[i];
+  }
+  return RD_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------ debug hooks --
+namespace {
+
+struct DebugCtx {
+  rdh::BuiltScene built;
+  rdh::DevNode *nodes = nullptr;
+  rdh::DevPrim *prims = nullptr;
+  rdh::DevObject *objects = nullptr;
+  rdh::DevJob *job = nullptr;
+  std::vector<void *> extra;  // every device allocation above, freed once
+  ~DebugCtx() {
+    for (void *p : extra) (void)hipFree(p);
+  }
+  template <class T> T *dev(const T *host, size_t n) {
+    T *d = nullptr;
+    if (hipMalloc(&d, sizeof(T) * std::max<size_t>(n, 1)) != hipSuccess) return nullptr;
+    extra.push_back(d);
+    if (host && n && hipMemcpy(d, host, sizeof(T) * n, hipMemcpyHostToDevice) != hipSuccess) return nullptr;
+    return d;
+  }
+};
+
+rd_status debug_setup(const rd_scene *scene, int leaf, DebugCtx *c) {
+  std::string err;
+  if (!scene) { rdh::set_global_error("scene is NULL"); return RD_ERR_INVALID_ARGUMENT; }
+  if (!rdh::build_scene(*scene, leaf, &c->built, &err)) {
+    rdh::set_global_error(err);
+    return RD_ERR_INVALID_ARGUMENT;
+  }
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+    (void)hipGetLastError();
+    rdh::set_global_error("no HIP device available");
+    return RD_ERR_DEVICE;
+  }
+  auto &b = c->built;
+  if (b.nodes.size()) c->nodes = c->dev(b.nodes.data(), b.nodes.size());
+  if (b.prims.size()) c->prims = c->dev(b.prims.data(), b.prims.size());
+  if (b.objects.size()) c->objects = c->dev(b.objects.data(), b.objects.size());
+  rdh::DeviceScene ds;
+  ds.nodes = c->nodes;
+  ds.prims = c->prims;
+  ds.objects = c->objects;
+  ds.n_nodes = (uint32_t)b.nodes.size();
+  ds.n_prims = (uint32_t)b.prims.size();
+  ds.r0 = b.r0;
+  rdh::DevJob j;
+  rdh::fill_job(*scene, ds, nullptr, 0, 0, &j);
+  c->job = nullptr;
+  c->job = c->dev(&j, 1);
+  if (!c->job) { rdh::set_global_error("device allocation failed"); return RD_ERR_OUT_OF_MEMORY; }
+  return RD_OK;
+}
+
+rd_status finish(hipError_t e) {
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e != hipSuccess) {
+    rdh::set_global_error(hipGetErrorString(e));
+    return RD_ERR_DEVICE;
+  }
+  return RD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+rd_status rd_debug_trace(const rd_scene *scene, int leaf, int brute, const float *rays, uint32_t n,
+                         int32_t *ids, float *ts, float *normals, uint8_t *front) {
+  if (!rays || !ids || !ts || !normals || !front) return RD_ERR_INVALID_ARGUMENT;
+  DebugCtx c;
+  rd_status st = debug_setup(scene, leaf, &c);
+  if (st != RD_OK) return st;
+  const float *d_rays = c.dev(rays, 6ull * n);
+  int32_t *d_ids = c.dev<int32_t>(nullptr, n);
+  float *d_ts = c.dev<float>(nullptr, n);
+  float *d_n = c.dev<float>(nullptr, 3ull * n);
+  uint8_t *d_f = c.dev<uint8_t>(nullptr, n);
+  if (!d_rays || !d_ids || !d_ts || !d_n || !d_f) return RD_ERR_OUT_OF_MEMORY;
+  hipError_t e = rdh::launch_debug_trace(c.nodes, c.prims, (uint32_t)c.built.prims.size(),
+                                         (uint32_t)c.built.nodes.size(), c.built.r0, c.objects,
+                                         d_rays, n, brute, d_ids, d_ts, d_n, d_f, nullptr);
+  if ((st = finish(e)) != RD_OK) return st;
+  if (n) {
+    (void)hipMemcpy(ids, d_ids, 4ull * n, hipMemcpyDeviceToHost);
+    (void)hipMemcpy(ts, d_ts, 4ull * n, hipMemcpyDeviceToHost);
+    (void)hipMemcpy(normals, d_n, 12ull * n, hipMemcpyDeviceToHost);
+    (void)hipMemcpy(front, d_f, n, hipMemcpyDeviceToHost);
+  }
+  return finish(hipSuccess);
+}
+
+rd_status rd_debug_scatter(const rd_scene *scene, const float *in, const uint32_t *bits,
+                           uint32_t n, float *out) {
+  if (!in || !bits || !out) return RD_ERR_INVALID_ARGUMENT;
+  for (uint32_t i = 0; i < n; ++i) {
+    uint32_t id;
+    std::memcpy(&id, in + 16ull * i + 7, 4);
+    if (!scene || id >= scene->object_count) {
+      rdh::set_global_error("rd_debug_scatter: object id out of range");
+      return RD_ERR_INVALID_ARGUMENT;
+    }
+  }
+  DebugCtx c;
+  rd_status st = debug_setup(scene, 4, &c);
+  if (st != RD_OK) return st;
+  const float *d_in = c.dev(in, 16ull * n);
+  const uint32_t *d_bits = c.dev(bits, 12ull * n);
+  float *d_out = c.dev<float>(nullptr, 16ull * n);
+  if (!d_in || !d_bits || !d_out) return RD_ERR_OUT_OF_MEMORY;
+  if ((st = finish(rdh::launch_debug_scatter(c.objects, d_in, d_bits, n, d_out, nullptr))) != RD_OK)
+    return st;
+  if (n) (void)hipMemcpy(out, d_out, 64ull * n, hipMemcpyDeviceToHost);
+  return finish(hipSuccess);
+}
+
+rd_status rd_debug_camera_rays(const rd_scene *scene, const uint32_t *pixels, uint32_t n,
+                               float *out) {
+  if (!pixels || !out || !scene) return RD_ERR_INVALID_ARGUMENT;
+  const uint64_t np = (uint64_t)scene->camera.resolution_x * scene->camera.resolution_y;
+  for (uint32_t i = 0; i < n; ++i)
+    if (pixels[i] >= np) return RD_ERR_INVALID_ARGUMENT;
+  DebugCtx c;
+  rd_status st = debug_setup(scene, 4, &c);
+  if (st != RD_OK) return st;
+  const uint32_t *d_px = c.dev(pixels, n);
+  float *d_out = c.dev<float>(nullptr, 6ull * n);
+  if (!d_px || !d_out) return RD_ERR_OUT_OF_MEMORY;
+  if ((st = finish(rdh::launch_debug_camera(c.job, d_px, n, d_out, nullptr))) != RD_OK) return st;
+  if (n) (void)hipMemcpy(out, d_out, 24ull * n, hipMemcpyDeviceToHost);
+  return finish(hipSuccess);
+}
+
+rd_status rd_debug_world(const rd_scene *scene, const float *dirs, uint32_t n, float *out) {
+  if (!dirs || !out || !scene) return RD_ERR_INVALID_ARGUMENT;
+  DebugCtx c;
+  rd_status st = debug_setup(scene, 4, &c);
+  if (st != RD_OK) return st;
+  const float *d_in = c.dev(dirs, 3ull * n);
+  float *d_out = c.dev<float>(nullptr, 3ull * n);
+  if (!d_in || !d_out) return RD_ERR_OUT_OF_MEMORY;
+  if ((st = finish(rdh::launch_debug_world(c.job, d_in, n, d_out, nullptr))) != RD_OK) return st;
+  if (n) (void)hipMemcpy(out, d_out, 12ull * n, hipMemcpyDeviceToHost);
+  return finish(hipSuccess);
+}
+
+rd_status rd_debug_philox(const uint32_t *in, uint32_t n, uint32_t *out) {
+  if (!in || !out) return RD_ERR_INVALID_ARGUMENT;
+  int nd = 0;
+  if (hipGetDeviceCount(&nd) != hipSuccess || nd == 0) {
+    (void)hipGetLastError();
+    rdh::set_global_error("no HIP device available");
+    return RD_ERR_DEVICE;
+  }
+  DebugCtx c;
+  const uint32_t *d_in = c.dev(in, 6ull * n);
+  uint32_t *d_out = c.dev<uint32_t>(nullptr, 4ull * n);
+  if (!d_in || !d_out) return RD_ERR_OUT_OF_MEMORY;
+  rd_status st = finish(rdh::launch_debug_philox(d_in, n, d_out, nullptr));
+  if (st != RD_OK) return st;
+  if (n) (void)hipMemcpy(out, d_out, 16ull * n, hipMemcpyDeviceToHost);
+  return finish(hipSuccess);
+}
+
+rd_status rd_debug_host_bvh_info(const rd_scene *scene, int leaf, uint32_t *n_nodes,
+                                 uint32_t *depth, char *err, size_t cap) {
+  if (!scene || !n_nodes || !depth) return RD_ERR_INVALID_ARGUMENT;
+  rdh::BuiltScene b;
+  std::string e;
+  bool ok = rdh::build_scene(*scene, leaf, &b, &e) && rdh::check_bvh(b, &e);
+  *n_nodes = (uint32_t)b.nodes.size();
+  *depth = b.depth;
+  if (err && cap) {
+    std::snprintf(err, cap, "%s", e.c_str());
+  }
+  return ok ? RD_OK : RD_ERR_INVALID_ARGUMENT;
+}
+
+rd_status rd_debug_host_trace(const rd_scene *scene, int leaf, const float *rays, uint32_t n,
+                              int32_t *ids, float *ts) {
+  if (!scene || !rays || !ids || !ts) return RD_ERR_INVALID_ARGUMENT;
+  rdh::BuiltScene b;
+  std::string e;
+  if (!rdh::build_scene(*scene, leaf, &b, &e)) {
+    rdh::set_global_error(e);
+    return RD_ERR_INVALID_ARGUMENT;
+  }
+  for (uint32_t i = 0; i < n; ++i) ids[i] = rdh::host_trace(b, rays + 6ull * i, rays + 6ull * i + 3, ts + i);
+  return RD_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,414 @@
+// Host-side scene digestion and binned-SAH BVH build (see bvh.hpp).
+//
+// Padding.  hit_sphere (cpu.rs:34-62) forms c = |o|^2 - 2 o.C + |C|^2 - r^2 in
+// FP32.  Its rounding error is at most ~17u L^2 with L = |o| + |C| + r and
+// u = 2^-24 (k and the discriminant add less), so the FP32 test can report a
+// hit for rays passing up to sqrt(r^2 + 17u L^2) - r outside the true sphere,
+// and the hit point it reports lies within that distance of the sphere.  Such
+// "phantom" hits are part of the reference's answer, so every primitive box is
+// grown by twice that excess (2e-6 L^2 inside the square root) for every ray
+// origin with |o|_1 <= r0, plus 1e-6 L for the box test's own rounding.  Rays
+// starting beyond r0 (only possible after hitting a primitive larger than the
+// rest of the scene, e.g. a ground sphere's far side) add a per-ray margin of
+// 2e-3 |o|_1 at traversal time (ray_tolerance in pathmath.hpp).  The CPU tests
+// compare this BVH against the linear scan on adversarial rays.
+#include "bvh.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+
+#include "pathmath.hpp"
+
+namespace rdh {
+
+namespace {
+
+struct Item {
+  double lo[3], hi[3], c[3];
+  uint32_t id;
+};
+
+struct Box {
+  double lo[3], hi[3];
+  Box() {
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = std::numeric_limits<double>::infinity();
+      hi[a] = -std::numeric_limits<double>::infinity();
+    }
+  }
+  void grow(const double *l, const double *h) {
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = std::min(lo[a], l[a]);
+      hi[a] = std::max(hi[a], h[a]);
+    }
+  }
+  double area() const {
+    double e[3];
+    for (int a = 0; a < 3; ++a) e[a] = std::max(0.0, hi[a] - lo[a]);
+    return 2.0 * (e[0] * e[1] + e[1] * e[2] + e[2] * e[0]);
+  }
+};
+
+double l1(const float *v) { return std::fabs((double)v[0]) + std::fabs((double)v[1]) + std::fabs((double)v[2]); }
+
+// sqrt(r^2 + 2e-6 L^2) - r, computed without cancellation.
+double sphere_excess(double r, double L) {
+  const double q = 2e-6 * L * L;
+  return q / (std::sqrt(r * r + q) + r);
+}
+
+struct Builder {
+  std::vector<Item> items;
+  std::vector<DevNode> nodes;
+  std::vector<uint32_t> order;
+  int leaf_size = 4;
+  uint32_t max_depth_seen = 0;
+  double sah = 0.0;
+  static constexpr int kBins = 16;
+  static constexpr int kMaxDepth = kStackDepth - 2;
+
+  static int ceil_log2(size_t n) {
+    int k = 0;
+    while (((size_t)1 << k) < n) ++k;
+    return k;
+  }
+
+  // Returns the child code for items[b, e).
+  int32_t build(size_t b, size_t e, uint32_t depth, const Box &bounds) {
+    const size_t n = e - b;
+    if (n <= (size_t)leaf_size) {
+      if (n == 1 || !worth_splitting(b, e, bounds)) return make_leaf(b, e, depth);
+    }
+    size_t mid = split(b, e, depth, bounds);
+    Box lb, rb;
+    for (size_t i = b; i < mid; ++i) lb.grow(items[i].lo, items[i].hi);
+    for (size_t i = mid; i < e; ++i) rb.grow(items[i].lo, items[i].hi);
+    const uint32_t idx = (uint32_t)nodes.size();
+    nodes.emplace_back();
+    max_depth_seen = std::max(max_depth_seen, depth);
+    sah += bounds.area();
+    const int32_t c0 = build(b, mid, depth + 1, lb);
+    const int32_t c1 = build(mid, e, depth + 1, rb);
+    DevNode &nd = nodes[idx];
+    std::memset(&nd, 0, sizeof(nd));
+    set_box(nd, 0, lb);
+    set_box(nd, 1, rb);
+    nd.child[0] = c0;
+    nd.child[1] = c1;
+    return (int32_t)idx;
+  }
+
+  static float down(double v) {
+    float f = (float)v;
+    if ((double)f > v) f = std::nextafter(f, -std::numeric_limits<float>::infinity());
+    return f;
+  }
+  static float up(double v) {
+    float f = (float)v;
+    if ((double)f < v) f = std::nextafter(f, std::numeric_limits<float>::infinity());
+    return f;
+  }
+  static void set_box(DevNode &nd, int k, const Box &b) {
+    nd.lo_x[k] = down(b.lo[0]); nd.hi_x[k] = up(b.hi[0]);
+    nd.lo_y[k] = down(b.lo[1]); nd.hi_y[k] = up(b.hi[1]);
+    nd.lo_z[k] = down(b.lo[2]); nd.hi_z[k] = up(b.hi[2]);
+  }
+
+  int32_t make_leaf(size_t b, size_t e, uint32_t depth) {
+    const uint32_t first = (uint32_t)order.size();
+    for (size_t i = b; i < e; ++i) order.push_back(items[i].id);
+    max_depth_seen = std::max(max_depth_seen, depth);
+    return leaf_code(first, (uint32_t)(e - b));
+  }
+
+  bool worth_splitting(size_t b, size_t e, const Box &bounds) {
+    // leaf cost n (one primitive test each) vs the best binned split,
+    // 1 + (A_l n_l + A_r n_r) / A (one node visit costs about one test)
+    double best;
+    best_split(b, e, bounds, &best, nullptr, nullptr);
+    return best < (double)(e - b);
+  }
+
+  // Best binned SAH split; returns cost, axis and bin boundary.
+  void best_split(size_t b, size_t e, const Box &bounds, double *cost, int *axis_out,
+                  double *plane_out) {
+    Box cb;
+    for (size_t i = b; i < e; ++i) cb.grow(items[i].c, items[i].c);
+    *cost = std::numeric_limits<double>::infinity();
+    const double area = std::max(bounds.area(), 1e-30);
+    for (int a = 0; a < 3; ++a) {
+      const double ext = cb.hi[a] - cb.lo[a];
+      if (!(ext > 0.0)) continue;
+      Box bb[kBins];
+      size_t cnt[kBins] = {0};
+      for (size_t i = b; i < e; ++i) {
+        int k = (int)((items[i].c[a] - cb.lo[a]) / ext * kBins);
+        k = std::min(std::max(k, 0), kBins - 1);
+        cnt[k]++;
+        bb[k].grow(items[i].lo, items[i].hi);
+      }
+      Box left[kBins];
+      size_t lc[kBins];
+      Box acc;
+      size_t ac = 0;
+      for (int k = 0; k < kBins; ++k) {
+        acc.grow(bb[k].lo, bb[k].hi);
+        ac += cnt[k];
+        left[k] = acc;
+        lc[k] = ac;
+      }
+      Box racc;
+      size_t rc = 0;
+      for (int k = kBins - 1; k > 0; --k) {
+        racc.grow(bb[k].lo, bb[k].hi);
+        rc += cnt[k];
+        if (lc[k - 1] == 0 || rc == 0) continue;
+        const double c = 1.0 + (left[k - 1].area() * lc[k - 1] + racc.area() * rc) / area;
+        if (c < *cost) {
+          *cost = c;
+          if (axis_out) *axis_out = a;
+          if (plane_out) *plane_out = cb.lo[a] + ext * k / kBins;
+        }
+      }
+    }
+  }
+
+  size_t split(size_t b, size_t e, uint32_t depth, const Box &bounds) {
+    const size_t n = e - b;
+    const int need = ceil_log2((n + leaf_size - 1) / leaf_size);
+    bool median = (int)depth + need + 1 >= kMaxDepth;
+    int axis = -1;
+    double plane = 0.0, cost;
+    if (!median) {
+      best_split(b, e, bounds, &cost, &axis, &plane);
+      if (axis < 0) median = true;
+    }
+    if (!median) {
+      auto it = std::partition(items.begin() + b, items.begin() + e,
+                               [&](const Item &x) { return x.c[axis] < plane; });
+      size_t mid = (size_t)(it - items.begin());
+      if (mid > b && mid < e) return mid;
+    }
+    // Median split on the widest centroid axis (also for coincident centroids).
+    Box cb;
+    for (size_t i = b; i < e; ++i) cb.grow(items[i].c, items[i].c);
+    int a = 0;
+    for (int k = 1; k < 3; ++k)
+      if (cb.hi[k] - cb.lo[k] > cb.hi[a] - cb.lo[a]) a = k;
+    size_t mid = b + n / 2;
+    std::nth_element(items.begin() + b, items.begin() + mid, items.begin() + e,
+                     [&](const Item &x, const Item &y) {
+                       return x.c[a] < y.c[a] || (x.c[a] == y.c[a] && x.id < y.id);
+                     });
+    return mid;
+  }
+};
+
+bool finite3(const float *v) { return std::isfinite(v[0]) && std::isfinite(v[1]) && std::isfinite(v[2]); }
+
+}  // namespace
+
+bool build_scene(const rd_scene &scene, int leaf_size, BuiltScene *out, std::string *err) {
+  const uint32_t n = scene.object_count;
+  if (n > 0 && scene.objects == nullptr) {
+    *err = "scene.objects is NULL with object_count > 0";
+    return false;
+  }
+  if (n >= (1u << 27)) {
+    *err = "too many objects (limit 2^27)";
+    return false;
+  }
+  if (leaf_size < 1 || leaf_size > kMaxLeafSize) {
+    *err = "leaf size must be in 1..8";
+    return false;
+  }
+  out->objects.resize(n);
+  out->prims.clear();
+  out->nodes.clear();
+  // Scene radius for the padding rule (L1 norms bound L2 norms).
+  double r_all = l1(scene.camera.position), r_small = r_all;
+  for (uint32_t i = 0; i < n; ++i) {
+    const rd_object &o = scene.objects[i];
+    if (o.kind != RD_SPHERE && o.kind != RD_CUBE) {
+      *err = "object " + std::to_string(i) + ": unknown geometry kind";
+      return false;
+    }
+    if (!finite3(o.center) || !std::isfinite(o.size)) {
+      *err = "object " + std::to_string(i) + ": non-finite geometry";
+      return false;
+    }
+    r_all = std::max(r_all, l1(o.center) + 2.0 * std::fabs((double)o.size));
+  }
+  for (uint32_t i = 0; i < n; ++i) {
+    const rd_object &o = scene.objects[i];
+    const double ext = l1(o.center) + 2.0 * std::fabs((double)o.size);
+    if (std::fabs((double)o.size) <= 0.25 * r_all) r_small = std::max(r_small, ext);
+  }
+  out->r0 = (float)r_small;
+
+  Builder bld;
+  bld.leaf_size = leaf_size;
+  bld.items.resize(n);
+  for (uint32_t i = 0; i < n; ++i) {
+    const rd_object &o = scene.objects[i];
+    DevObject &d = out->objects[i];
+    std::memset(&d, 0, sizeof(d));
+    d.geo[0] = o.center[0]; d.geo[1] = o.center[1]; d.geo[2] = o.center[2]; d.geo[3] = o.size;
+    d.m0[0] = o.albedo[0]; d.m0[1] = o.albedo[1]; d.m0[2] = o.albedo[2];
+    d.m0[3] = o.roughness * o.roughness;                         // cpu.rs:262-263
+    d.m1[0] = o.emission_color[0] * o.emission_strength;         // cpu.rs:332-333
+    d.m1[1] = o.emission_color[1] * o.emission_strength;
+    d.m1[2] = o.emission_color[2] * o.emission_strength;
+    d.m1[3] = o.metallic;
+    d.m2[0] = o.transmission;
+    d.m2[1] = o.ior;
+    d.m2[2] = bitsf(o.kind);
+
+    Item &it = bld.items[i];
+    it.id = i;
+    const double size = std::fabs((double)o.size);
+    const double huge = size > 0.25 * r_all;
+    const double L = (huge ? r_all : r_small) + l1(o.center) + size;
+    double half, pad;
+    if (o.kind == RD_SPHERE) {
+      half = size;
+      pad = 2.0 * sphere_excess(size, L);
+    } else {
+      half = 0.5 * size;
+      pad = 2e-5 * L;
+    }
+    pad += 1e-6 * (r_all + l1(o.center) + size) + 1e-7;
+    for (int a = 0; a < 3; ++a) {
+      it.c[a] = o.center[a];
+      it.lo[a] = (double)o.center[a] - half - pad;
+      it.hi[a] = (double)o.center[a] + half + pad;
+    }
+  }
+  if (n > 0) {
+    Box root;
+    for (auto &it : bld.items) root.grow(it.lo, it.hi);
+    if (n <= (uint32_t)leaf_size) {
+      // a root node whose second child is an empty leaf
+      bld.nodes.emplace_back();
+      DevNode &nd = bld.nodes[0];
+      std::memset(&nd, 0, sizeof(nd));
+      const int32_t leaf = bld.make_leaf(0, n, 1);
+      Builder::set_box(nd, 0, root);
+      Builder::set_box(nd, 1, root);
+      nd.child[0] = leaf;
+      nd.child[1] = leaf_code(0, 0);
+      bld.max_depth_seen = 1;
+    } else {
+      bld.build(0, n, 1, root);
+    }
+  }
+  out->nodes = std::move(bld.nodes);
+  out->depth = bld.max_depth_seen;
+  out->sah_cost = bld.sah;
+  out->prims.resize(n);
+  for (uint32_t k = 0; k < n; ++k) {
+    const uint32_t id = bld.order[k];
+    const rd_object &o = scene.objects[id];
+    DevPrim &p = out->prims[k];
+    std::memset(&p, 0, sizeof(p));
+    if (o.kind == RD_SPHERE) {
+      p.a[0] = o.center[0]; p.a[1] = o.center[1]; p.a[2] = o.center[2];
+      // dot(C, C) and r*r in the reference's FP32 order (cpu.rs:42-43)
+      p.a[3] = o.center[0] * o.center[0] + o.center[1] * o.center[1] + o.center[2] * o.center[2];
+      p.b[0] = o.size * o.size;
+      p.b[3] = bitsf(id);
+    } else {
+      // cpu.rs:65-71: half = side * 0.5, min = C - half, max = C + half
+      const float h = o.size * 0.5f;
+      for (int a = 0; a < 3; ++a) {
+        p.a[a] = o.center[a] - h;
+        p.b[a] = o.center[a] + h;
+      }
+      p.b[3] = bitsf(id | kTagCube);
+    }
+  }
+  return true;
+}
+
+bool check_bvh(const BuiltScene &b, std::string *err) {
+  const size_t n = b.prims.size();
+  if (n == 0) return true;
+  if (b.depth + 1 > (uint32_t)kStackDepth) {
+    *err = "BVH deeper than the traversal stack";
+    return false;
+  }
+  std::vector<int> seen(n, 0);
+  for (size_t i = 0; i < b.nodes.size(); ++i) {
+    const DevNode &nd = b.nodes[i];
+    for (int k = 0; k < 2; ++k) {
+      const int32_t c = nd.child[k];
+      if (is_leaf(c)) {
+        const uint32_t f = leaf_first(c), cnt = leaf_count(c);
+        if (f + cnt > n || cnt > (uint32_t)kMaxLeafSize) {
+          *err = "leaf out of range";
+          return false;
+        }
+        for (uint32_t j = f; j < f + cnt; ++j) seen[j]++;
+      } else if (c <= (int32_t)i || (size_t)c >= b.nodes.size()) {
+        *err = "child index not after its parent";
+        return false;
+      }
+    }
+  }
+  for (size_t j = 0; j < n; ++j)
+    if (seen[j] != 1) {
+      *err = "primitive " + std::to_string(j) + " referenced " + std::to_string(seen[j]) + " times";
+      return false;
+    }
+  return true;
+}
+
+int32_t host_trace(const BuiltScene &b, const float o[3], const float d[3], float *t_out) {
+  const RayPre r = ray_pre(f3(o[0], o[1], o[2]), f3(d[0], d[1], d[2]));
+  float bt = 0.f;
+  uint32_t bid = kNoHit;
+  const bool finite = std::isfinite(d[0]) && std::isfinite(d[1]) && std::isfinite(d[2]) &&
+                      std::isfinite(o[0]) && std::isfinite(o[1]) && std::isfinite(o[2]);
+  if (!finite || b.nodes.empty()) {
+    for (const DevPrim &p : b.prims) test_prim(r, p, &bt, &bid);
+  } else {
+    const SlabRay s = slab_ray(r, ray_tolerance(r, b.r0));
+    int32_t stack[kStackDepth];
+    int sp = 0;
+    int32_t node = 0;
+    for (;;) {
+      const DevNode &nd = b.nodes[node];
+      const float tc = cull_limit(bt, bid);
+      float t0, t1;
+      bool h0 = slab_hit(s, nd.lo_x[0], nd.hi_x[0], nd.lo_y[0], nd.hi_y[0], nd.lo_z[0], nd.hi_z[0], tc, &t0);
+      bool h1 = slab_hit(s, nd.lo_x[1], nd.hi_x[1], nd.lo_y[1], nd.hi_y[1], nd.lo_z[1], nd.hi_z[1], tc, &t1);
+      const int32_t c0 = nd.child[0], c1 = nd.child[1];
+      if (h0 && is_leaf(c0)) {
+        for (uint32_t j = leaf_first(c0), e = j + leaf_count(c0); j < e; ++j) test_prim(r, b.prims[j], &bt, &bid);
+        h0 = false;
+      }
+      if (h1 && is_leaf(c1)) {
+        for (uint32_t j = leaf_first(c1), e = j + leaf_count(c1); j < e; ++j) test_prim(r, b.prims[j], &bt, &bid);
+        h1 = false;
+      }
+      if (h0 && h1) {
+        const bool first0 = t0 <= t1;
+        stack[sp++] = first0 ? c1 : c0;
+        node = first0 ? c0 : c1;
+      } else if (h0) {
+        node = c0;
+      } else if (h1) {
+        node = c1;
+      } else {
+        if (sp == 0) break;
+        node = stack[--sp];
+      }
+    }
+  }
+  *t_out = bid == kNoHit ? 0.f : bt;
+  return bid == kNoHit ? -1 : (int32_t)bid;
+}
+
+}  // namespace rdh

@@ -1,0 +1,319 @@
+// Wavefront engine (see engine.hpp).
+#include "engine.hpp"
+
+#include <algorithm>
+#include <cstring>
+
+namespace rdh {
+
+Engine::~Engine() {
+  release_buffers();
+  for (hipEvent_t e : ev_pool_) (void)hipEventDestroy(e);
+  if (counters) (void)hipFree(counters);
+  if (stream) (void)hipStreamDestroy(stream);
+}
+
+bool Engine::check(hipError_t e, const char *what, std::string *err) {
+  if (e == hipSuccess) return true;
+  *err = std::string(what) + ": " + hipGetErrorString(e);
+  return false;
+}
+
+bool Engine::init(std::string *err) {
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n == 0) {
+    *err = std::string("no HIP device available (") + hipGetErrorString(e) +
+           "); the HIP renderer needs an MI355X (gfx950) GPU";
+    (void)hipGetLastError();
+    return false;
+  }
+  if (!check(hipSetDevice(device), "hipSetDevice", err)) return false;
+  hipDeviceProp_t prop;
+  if (!check(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties", err)) return false;
+  num_cus = prop.multiProcessorCount;
+  if (!check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate", err))
+    return false;
+  if (!check(hipMalloc(&counters, sizeof(DevCounters)), "hipMalloc(counters)", err)) return false;
+  if (!check(hipMemsetAsync(counters, 0, sizeof(DevCounters), stream), "hipMemset", err))
+    return false;
+  if (!check(hipHostMalloc(&ctl_host, sizeof(QueueCtl), hipHostMallocDefault), "hipHostMalloc",
+             err))
+    return false;
+  if (!check(hipMalloc(&ctl, sizeof(QueueCtl)), "hipMalloc(ctl)", err)) return false;
+  return check(hipStreamSynchronize(stream), "hipStreamSynchronize", err);
+}
+
+void Engine::release_buffers() {
+  for (int p = 0; p < 2; ++p) {
+    float **f[] = {&q[p].ox, &q[p].oy, &q[p].oz, &q[p].dx, &q[p].dy, &q[p].dz,
+                   &q[p].ax, &q[p].ay, &q[p].az, &q[p].lx, &q[p].ly, &q[p].lz};
+    for (float **x : f)
+      if (*x) { (void)hipFree(*x); *x = nullptr; }
+    if (q[p].tb) { (void)hipFree(q[p].tb); q[p].tb = nullptr; }
+    if (q[p].pidx) { (void)hipFree(q[p].pidx); q[p].pidx = nullptr; }
+  }
+  if (hit_t) { (void)hipFree(hit_t); hit_t = nullptr; }
+  if (hit_id) { (void)hipFree(hit_id); hit_id = nullptr; }
+  if (L) { (void)hipFree(L); L = nullptr; }
+  q_alloc = 0;
+  l_alloc = 0;
+  if (d_tickets) { (void)hipFree(d_tickets); d_tickets = nullptr; }
+  if (d_ticket_done) { (void)hipFree(d_ticket_done); d_ticket_done = nullptr; }
+  ticket_alloc = 0;
+  if (ctl) { (void)hipFree(ctl); ctl = nullptr; }
+  if (ctl_host) { (void)hipHostFree(ctl_host); ctl_host = nullptr; }
+}
+
+bool Engine::ensure_buffers(std::string *err) {
+  if (opt.queue < 1024) opt.queue = 1024;
+  if (q_alloc != opt.queue) {
+    for (int p = 0; p < 2; ++p) {
+      float **f[] = {&q[p].ox, &q[p].oy, &q[p].oz, &q[p].dx, &q[p].dy, &q[p].dz,
+                     &q[p].ax, &q[p].ay, &q[p].az, &q[p].lx, &q[p].ly, &q[p].lz};
+      for (float **x : f) {
+        if (*x) (void)hipFree(*x);
+        *x = nullptr;
+        if (!check(hipMalloc(x, sizeof(float) * opt.queue), "hipMalloc(queue)", err)) return false;
+      }
+      if (q[p].tb) (void)hipFree(q[p].tb);
+      if (q[p].pidx) (void)hipFree(q[p].pidx);
+      q[p].tb = q[p].pidx = nullptr;
+      if (!check(hipMalloc(&q[p].tb, 4ull * opt.queue), "hipMalloc(queue)", err)) return false;
+      if (!check(hipMalloc(&q[p].pidx, 4ull * opt.queue), "hipMalloc(queue)", err)) return false;
+    }
+    if (hit_t) (void)hipFree(hit_t);
+    if (hit_id) (void)hipFree(hit_id);
+    hit_t = nullptr;
+    hit_id = nullptr;
+    if (!check(hipMalloc(&hit_t, 4ull * opt.queue), "hipMalloc(hits)", err)) return false;
+    if (!check(hipMalloc(&hit_id, 4ull * opt.queue), "hipMalloc(hits)", err)) return false;
+    q_alloc = opt.queue;
+  }
+  if (l_alloc < opt.l_capacity) {
+    if (L) (void)hipFree(L);
+    L = nullptr;
+    if (!check(hipMalloc(&L, sizeof(float4) * opt.l_capacity), "hipMalloc(radiance arena)", err))
+      return false;
+    l_alloc = opt.l_capacity;
+  }
+  if (!ctl && !check(hipMalloc(&ctl, sizeof(QueueCtl)), "hipMalloc(ctl)", err)) return false;
+  if (!ctl_host &&
+      !check(hipHostMalloc(&ctl_host, sizeof(QueueCtl), hipHostMallocDefault), "hipHostMalloc", err))
+    return false;
+  return true;
+}
+
+bool Engine::ensure_tickets(uint32_t n, std::string *err) {
+  if (n <= ticket_alloc) return true;
+  uint32_t cap = std::max<uint32_t>(n, 2 * ticket_alloc);
+  if (d_tickets) (void)hipFree(d_tickets);
+  if (d_ticket_done) (void)hipFree(d_ticket_done);
+  d_tickets = nullptr;
+  d_ticket_done = nullptr;
+  if (!check(hipMalloc(&d_tickets, sizeof(DevTicket) * cap), "hipMalloc(tickets)", err)) return false;
+  if (!check(hipMalloc(&d_ticket_done, 8ull * cap), "hipMalloc(tickets)", err)) return false;
+  ticket_alloc = cap;
+  return true;
+}
+
+bool Engine::upload_scene(const BuiltScene &b, DeviceScene *out, std::string *err) {
+  free_scene(out);
+  out->n_nodes = (uint32_t)b.nodes.size();
+  out->n_prims = (uint32_t)b.prims.size();
+  out->r0 = b.r0;
+  out->depth = b.depth;
+  const size_t nb = sizeof(DevNode) * b.nodes.size(), pb = sizeof(DevPrim) * b.prims.size(),
+               ob = sizeof(DevObject) * b.objects.size();
+  out->bytes = nb + pb + ob;
+  if (nb && !check(hipMalloc(&out->nodes, nb), "hipMalloc(bvh)", err)) return false;
+  if (pb && !check(hipMalloc(&out->prims, pb), "hipMalloc(prims)", err)) return false;
+  if (ob && !check(hipMalloc(&out->objects, ob), "hipMalloc(objects)", err)) return false;
+  if (nb && !check(hipMemcpyAsync(out->nodes, b.nodes.data(), nb, hipMemcpyHostToDevice, stream),
+                   "upload bvh", err))
+    return false;
+  if (pb && !check(hipMemcpyAsync(out->prims, b.prims.data(), pb, hipMemcpyHostToDevice, stream),
+                   "upload prims", err))
+    return false;
+  if (ob && !check(hipMemcpyAsync(out->objects, b.objects.data(), ob, hipMemcpyHostToDevice,
+                                  stream),
+                   "upload objects", err))
+    return false;
+  return check(hipStreamSynchronize(stream), "upload scene", err);
+}
+
+void Engine::free_scene(DeviceScene *s) {
+  if (s->nodes) (void)hipFree(s->nodes);
+  if (s->prims) (void)hipFree(s->prims);
+  if (s->objects) (void)hipFree(s->objects);
+  *s = DeviceScene();
+}
+
+IterArgs Engine::iter_args(const DevJob *d_jobs, uint32_t n_tickets) const {
+  IterArgs a;
+  a.q[0] = q[0];
+  a.q[1] = q[1];
+  a.hit_t = hit_t;
+  a.hit_id = hit_id;
+  a.ctl = ctl;
+  a.jobs = d_jobs;
+  a.tickets = d_tickets;
+  a.n_tickets = n_tickets;
+  a.capacity = q_alloc;
+  a.L = L;
+  a.ticket_done = d_ticket_done;
+  a.counters = counters;
+  a.brute_force = opt.brute_force;
+  return a;
+}
+
+LaunchShape Engine::shape(int which) const {
+  LaunchShape s;
+  s.block = opt.block;
+  if (opt.grid > 0) {
+    s.grid = opt.grid;
+  } else {
+    const int per_cu = std::max(1, occupancy_blocks(which, s.block));
+    s.grid = num_cus * per_cu;
+  }
+  return s;
+}
+
+bool Engine::timed(int which, std::string *err,
+                   hipError_t (*fn)(const IterArgs &, int, const LaunchShape &, bool, hipStream_t),
+                   const IterArgs &a, int parity) {
+  const bool count = opt.count != 0;
+  if (!opt.kernel_timing || count) return check(fn(a, parity, shape(which), count, stream), "kernel launch", err);
+  hipEvent_t e0, e1;
+  if (!check(hipEventCreate(&e0), "hipEventCreate", err)) return false;
+  if (!check(hipEventCreate(&e1), "hipEventCreate", err)) return false;
+  if (!check(hipEventRecord(e0, stream), "hipEventRecord", err)) return false;
+  if (!check(fn(a, parity, shape(which), false, stream), "kernel launch", err)) return false;
+  if (!check(hipEventRecord(e1, stream), "hipEventRecord", err)) return false;
+  if (!check(hipEventSynchronize(e1), "hipEventSynchronize", err)) return false;
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  kernel_ms[which] += ms;
+  kernel_launches[which] += 1;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return true;
+}
+
+bool Engine::launch_iteration(const IterArgs &a, int parity, std::string *err) {
+  ++iterations;
+  return timed(0, err, launch_generate, a, parity) && timed(1, err, launch_extend, a, parity) &&
+         timed(2, err, launch_shade, a, parity);
+}
+
+bool Engine::run_tickets(const DevJob *d_jobs, std::vector<DevTicket> tickets,
+                         const std::vector<DevJob> &host_jobs, std::string *err) {
+  if (tickets.empty()) return true;
+  if (!ensure_buffers(err)) return false;
+  uint64_t w = 0;
+  uint32_t max_bounces = 1;
+  std::vector<DevTicket> traced;
+  for (auto &t : tickets) {
+    t.work_begin = w;
+    t.l_offset = w;
+    const DevJob &j = host_jobs[t.job];
+    if (j.max_bounces > 0) {
+      traced.push_back(t);
+      w += t.paths;
+      max_bounces = std::max(max_bounces, j.max_bounces);
+    }
+  }
+  if (w > l_alloc) {
+    *err = "pass needs " + std::to_string(w) + " radiance slots, arena holds " +
+           std::to_string(l_alloc);
+    return false;
+  }
+  if (!traced.empty()) {
+    const uint32_t n = (uint32_t)traced.size();
+    if (!ensure_tickets(n, err)) return false;
+    if (!check(hipMemcpyAsync(d_tickets, traced.data(), sizeof(DevTicket) * n,
+                              hipMemcpyHostToDevice, stream),
+               "upload tickets", err))
+      return false;
+    if (!check(hipMemsetAsync(d_ticket_done, 0, 8ull * n, stream), "hipMemset", err)) return false;
+    if (!check(launch_set_ctl(ctl, w, stream), "set ctl", err)) return false;
+    const IterArgs a = iter_args(d_jobs, n);
+    int parity = 0;
+    const uint64_t cap_iters = (w / q_alloc + 2) * (uint64_t)(max_bounces + 1) + 64;
+    uint64_t it = 0;
+    for (;;) {
+      const int group = std::max(1, opt.sync_every);
+      for (int g = 0; g < group; ++g) {
+        if (!launch_iteration(a, parity, err)) return false;
+        parity ^= 1;
+        ++it;
+      }
+      if (!check(hipMemcpyAsync(ctl_host, ctl, sizeof(QueueCtl), hipMemcpyDeviceToHost, stream),
+                 "read queue state", err))
+        return false;
+      if (!check(hipStreamSynchronize(stream), "iteration", err)) return false;
+      if (ctl_host->count[parity] == 0 && ctl_host->cursor[parity] == w) break;
+      if (it > cap_iters) {
+        *err = "wavefront loop made no progress (internal error)";
+        return false;
+      }
+    }
+  }
+  // Accumulate every ticket in order: per pixel, samples in sample order.
+  for (const auto &t : tickets) {
+    const DevJob &j = host_jobs[t.job];
+    const float4 *src = j.max_bounces > 0 ? L : nullptr;
+    DevTicket tt = t;
+    if (j.max_bounces == 0) tt.l_offset = 0;
+    const int grid = std::min<int>((int)((j.width * j.height + 255) / 256), num_cus * 8);
+    if (opt.kernel_timing && !opt.count) {
+      hipEvent_t e0, e1;
+      (void)hipEventCreate(&e0);
+      (void)hipEventCreate(&e1);
+      (void)hipEventRecord(e0, stream);
+      if (!check(launch_accumulate(d_jobs + t.job, tt, src, counters, false, std::max(grid, 1), stream),
+                 "accumulate", err))
+        return false;
+      (void)hipEventRecord(e1, stream);
+      (void)hipEventSynchronize(e1);
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      kernel_ms[3] += ms;
+      kernel_launches[3] += 1;
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+    } else if (!check(launch_accumulate(d_jobs + t.job, tt, src, counters, opt.count != 0,
+                                        std::max(grid, 1), stream),
+                      "accumulate", err)) {
+      return false;
+    }
+  }
+  return true;
+}
+
+void fill_job(const rd_scene &s, const DeviceScene &ds, float *acc, uint32_t max_bounces,
+              uint64_t seed, DevJob *j) {
+  std::memset(j, 0, sizeof(*j));
+  j->nodes = ds.nodes;
+  j->prims = ds.prims;
+  j->objects = ds.objects;
+  j->acc = acc;
+  j->n_prims = ds.n_prims;
+  j->n_nodes = ds.n_nodes;
+  j->width = s.camera.resolution_x;
+  j->height = s.camera.resolution_y;
+  j->max_bounces = max_bounces;
+  j->world_kind = s.world.kind;
+  j->seed = seed;
+  std::memcpy(j->inv_proj, s.camera.inverse_proj_matrix, sizeof(j->inv_proj));
+  std::memcpy(j->inv_view, s.camera.inverse_view_matrix, sizeof(j->inv_view));
+  for (int k = 0; k < 3; ++k) {
+    j->position[k] = s.camera.position[k];
+    j->top[k] = s.world.top_color[k];
+    j->bottom[k] = s.world.bottom_color[k];
+    j->solid[k] = s.world.solid_color[k];
+  }
+  j->r0_tol = ds.r0;
+}
+
+}  // namespace rdh

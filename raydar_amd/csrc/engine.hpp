@@ -1,0 +1,103 @@
+// Wavefront engine: owns the device path queues, the radiance arena, the
+// work-ticket list and the library's HIP stream, and drives iterations of
+// generate -> extend -> shade until every ticket's paths have finished.
+//
+// The same engine serves the per-scene renderer (one job, one ticket per
+// pass), the batched renderer (many jobs, tickets for all of them in one
+// work list) and the render server (tickets appended as requests arrive).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "bvh.hpp"
+#include "kernels.hpp"
+#include "layout.hpp"
+
+namespace rdh {
+
+// Device copy of a BuiltScene.
+struct DeviceScene {
+  DevNode *nodes = nullptr;
+  DevPrim *prims = nullptr;
+  DevObject *objects = nullptr;
+  uint32_t n_nodes = 0, n_prims = 0;
+  float r0 = 0.f;
+  uint32_t depth = 0;
+  size_t bytes = 0;
+};
+
+struct EngineOptions {
+  int block = 256;
+  uint32_t queue = 1u << 23;      // paths in flight (two ping-pong queues)
+  uint64_t l_capacity = 1ull << 27;  // radiance slots (16 B each)
+  int grid = 0;                   // 0: occupancy-derived
+  int brute_force = 0;
+  int leaf = 4;
+  int count = 0;
+  int kernel_timing = 0;
+  int sync_every = 4;             // iterations between completion checks
+};
+
+class Engine {
+ public:
+  ~Engine();
+  // Selects the device and creates the stream; allocation is lazy.
+  bool init(std::string *err);
+  bool ensure_buffers(std::string *err);  // (re)allocates for current options
+  void release_buffers();
+
+  bool upload_scene(const BuiltScene &b, DeviceScene *out, std::string *err);
+  void free_scene(DeviceScene *s);
+
+  // Runs one batch of tickets to completion (all paths finished), then
+  // accumulates every ticket into its job's accumulator in ticket order.
+  // `jobs` is the device job table (already uploaded).
+  bool run_tickets(const DevJob *d_jobs, std::vector<DevTicket> tickets,
+                   const std::vector<DevJob> &host_jobs, std::string *err);
+
+  bool check(hipError_t e, const char *what, std::string *err);
+
+  EngineOptions opt;
+  hipStream_t stream = nullptr;
+  int device = 0;
+  int num_cus = 0;
+  DevCounters *counters = nullptr;  // device
+  double kernel_ms[5] = {0, 0, 0, 0, 0};
+  uint64_t kernel_launches[5] = {0, 0, 0, 0, 0};
+  uint64_t iterations = 0;
+
+  // buffers (public so the server can drive iterations itself)
+  PathQueue q[2] = {};
+  float *hit_t = nullptr;
+  uint32_t *hit_id = nullptr;
+  QueueCtl *ctl = nullptr;
+  QueueCtl *ctl_host = nullptr;  // pinned
+  float4 *L = nullptr;
+  uint64_t l_alloc = 0;
+  uint32_t q_alloc = 0;
+  DevTicket *d_tickets = nullptr;
+  uint32_t ticket_alloc = 0;
+  unsigned long long *d_ticket_done = nullptr;
+
+  IterArgs iter_args(const DevJob *d_jobs, uint32_t n_tickets) const;
+  LaunchShape shape(int which) const;
+  bool launch_iteration(const IterArgs &a, int parity, std::string *err);
+  bool ensure_tickets(uint32_t n, std::string *err);
+
+ private:
+  bool timed(int which, std::string *err, hipError_t (*fn)(const IterArgs &, int,
+                                                             const LaunchShape &, bool,
+                                                             hipStream_t),
+             const IterArgs &a, int parity);
+  std::vector<hipEvent_t> ev_pool_;
+};
+
+// Fills the camera / world / render fields of a DevJob from a scene.
+void fill_job(const rd_scene &s, const DeviceScene &ds, float *acc, uint32_t max_bounces,
+              uint64_t seed, DevJob *j);
+
+}  // namespace rdh

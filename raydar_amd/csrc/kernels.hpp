@@ -1,0 +1,73 @@
+// Wavefront kernels of the HIP path tracer and their host-side launchers.
+//
+// One iteration of parity p (queues q[p] -> q[1-p]) is three launches:
+//   generate  appends new camera paths (cpu.rs:234-251) after the survivors
+//             in q[p], taking global work indices from the cursor
+//   extend    closest hit for every path in q[p] (cpu.rs:344-394) through the
+//             BVH with an LDS stack, or by linear scan
+//   shade     scatter or sky (cpu.rs:256-338); survivors are compacted into
+//             q[1-p] with wave64 ballot + mbcnt and one atomic per wave;
+//             finished paths write their radiance to the L arena
+// The launches read the queue sizes from device memory (QueueCtl), so the
+// host never waits between them and an iteration can be captured in a graph.
+// A pass is finished when the cursor reached the end of the work list and the
+// queue is empty; accumulate then adds each pixel's samples in sample order
+// (cpu.rs:204-209) and tonemap writes RGBA8 (cpu.rs:221-230).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "layout.hpp"
+
+namespace rdh {
+
+struct IterArgs {
+  PathQueue q[2];
+  float *hit_t;
+  uint32_t *hit_id;
+  QueueCtl *ctl;
+  const DevJob *jobs;
+  const DevTicket *tickets;
+  uint32_t n_tickets;
+  uint32_t capacity;
+  float4 *L;                        // radiance arena (rgb, 1)
+  unsigned long long *ticket_done;  // finished paths per ticket
+  DevCounters *counters;            // used by the counting build only
+  int brute_force;                  // 1: linear scan instead of the BVH
+};
+
+struct LaunchShape {
+  int block = 256;   // 64, 128 or 256
+  int grid = 0;      // workgroups per launch
+};
+
+hipError_t launch_generate(const IterArgs &a, int parity, const LaunchShape &s, bool count,
+                           hipStream_t st);
+hipError_t launch_extend(const IterArgs &a, int parity, const LaunchShape &s, bool count,
+                         hipStream_t st);
+hipError_t launch_shade(const IterArgs &a, int parity, const LaunchShape &s, bool count,
+                        hipStream_t st);
+hipError_t launch_accumulate(const DevJob *job, const DevTicket &t, const float4 *L,
+                             DevCounters *counters, bool count, int grid, hipStream_t st);
+hipError_t launch_tonemap(const DevJob *job, uint32_t n_pixels, float spp, uchar4 *out,
+                          DevCounters *counters, bool count, int grid, hipStream_t st);
+// Workgroups of kernel `which` (0 generate, 1 extend, 2 shade) that fit on one CU.
+int occupancy_blocks(int which, int block);
+
+hipError_t launch_set_ctl(QueueCtl *ctl, unsigned long long total, hipStream_t st);
+hipError_t launch_zero_acc(float *acc, uint64_t n_floats, hipStream_t st);
+
+// Parity-test entry kernels (single rays / single hits).
+hipError_t launch_debug_trace(const DevNode *nodes, const DevPrim *prims, uint32_t n_prims,
+                              uint32_t n_nodes, float r0, const DevObject *objects,
+                              const float *rays, uint32_t n, int brute, int32_t *ids, float *ts,
+                              float *normals, uint8_t *front, hipStream_t st);
+hipError_t launch_debug_scatter(const DevObject *objects, const float *in, const uint32_t *bits,
+                                uint32_t n, float *out, hipStream_t st);
+hipError_t launch_debug_camera(const DevJob *job, const uint32_t *pixels, uint32_t n, float *out,
+                               hipStream_t st);
+hipError_t launch_debug_world(const DevJob *job, const float *dirs, uint32_t n, float *out,
+                              hipStream_t st);
+hipError_t launch_debug_philox(const uint32_t *in, uint32_t n, uint32_t *out, hipStream_t st);
+
+}  // namespace rdh

@@ -1,0 +1,124 @@
+// Device memory layout of the wavefront path tracer (host + device view).
+//
+// Everything a kernel touches lives in HBM in one of these forms:
+//   DevNode    64 B  binary BVH node holding BOTH children's boxes, so one
+//                    node fetch (4 x 16 B) feeds two slab tests
+//   DevPrim    32 B  primitive in BVH leaf order, pre-digested for the exact
+//                    hit tests of src/renderer/cpu.rs:34-98
+//   DevObject  64 B  geometry + material by ORIGINAL object index (the index
+//                    the closest-hit tie rule and the shading use)
+//   DevJob           one scene being rendered: camera, world, scene buffers,
+//                    accumulator, RNG seed, bounce limit
+//   DevTicket        one pass of one job: `paths` consecutive path indices
+//                    s*W*H + p, s in [0, spp); radiance slots in the L arena
+//   Path queue       structure of arrays, 56 B per path in flight
+#pragma once
+
+#include <stdint.h>
+
+#ifndef __HIPCC__
+#ifndef __host__
+#define __host__
+#define __device__
+#endif
+#endif
+
+namespace rdh {
+
+constexpr int kMaxLeafSize = 8;
+constexpr int kStackDepth = 40;          // >= max BVH depth the builder emits
+constexpr uint32_t kNoHit = 0x7fffffffu; // best-id sentinel: "no hit yet"
+
+// Leaf child code: bit 31 set, first prim in bits 4..30, count in bits 0..3.
+__host__ __device__ inline bool is_leaf(int32_t c) { return c < 0; }
+__host__ __device__ inline int32_t leaf_code(uint32_t first, uint32_t count) {
+  return (int32_t)(0x80000000u | (first << 4) | count);
+}
+__host__ __device__ inline uint32_t leaf_first(int32_t c) { return ((uint32_t)c & 0x7fffffffu) >> 4; }
+__host__ __device__ inline uint32_t leaf_count(int32_t c) { return (uint32_t)c & 15u; }
+
+struct alignas(16) DevNode {
+  float lo_x[2], hi_x[2];  // child 0 / child 1 x slab  (lo0, lo1, hi0, hi1)
+  float lo_y[2], hi_y[2];
+  float lo_z[2], hi_z[2];
+  int32_t child[2];        // >= 0: internal node index; < 0: leaf code
+  uint32_t pad[2];
+};
+static_assert(sizeof(DevNode) == 64, "DevNode must be 64 bytes");
+
+// Primitive tag in DevPrim::b.w: original object id | kind << 31.
+struct alignas(16) DevPrim {
+  // sphere: a = (cx, cy, cz, dot(C,C)), b = (r*r, 0, 0, tag)
+  // cube:   a = (min x, y, z, 0),       b = (max x, y, z, tag)
+  float a[4];
+  float b[4];
+};
+static_assert(sizeof(DevPrim) == 32, "DevPrim must be 32 bytes");
+
+constexpr uint32_t kTagCube = 0x80000000u;
+
+struct alignas(16) DevObject {
+  float geo[4];   // center xyz, size (radius or side length)
+  float m0[4];    // albedo xyz, roughness^2 (cpu.rs:262-263)
+  float m1[4];    // emission_color * emission_strength xyz (cpu.rs:332-333), metallic
+  float m2[4];    // transmission, ior, kind (as uint bits), 0
+};
+static_assert(sizeof(DevObject) == 64, "DevObject must be 64 bytes");
+
+struct DevJob {
+  const DevNode *nodes;
+  const DevPrim *prims;
+  const DevObject *objects;
+  float *acc;               // W*H*4 RGBA32F accumulator (cpu.rs:113)
+  uint32_t n_prims;
+  uint32_t n_nodes;
+  uint32_t width, height;
+  uint32_t max_bounces;
+  uint32_t world_kind;
+  uint64_t seed;
+  float inv_proj[16];       // column-major, as stored in the scene (camera.rs:29)
+  float inv_view[16];       // column-major (camera.rs:28)
+  float position[4];
+  float top[4], bottom[4], solid[4];
+  float r0_tol;             // |o|_1 above which a ray needs the dynamic box margin
+  float pad_[3];
+};
+
+struct DevTicket {
+  uint64_t work_begin;      // first global work index of this ticket
+  uint64_t l_offset;        // first radiance slot in the L arena
+  uint32_t job;
+  uint32_t sample_base;     // global sample index of local sample 0
+  uint32_t spp;             // samples in this pass
+  uint32_t paths;           // spp * W * H
+};
+
+// Structure-of-arrays path queue (one of two ping-pong buffers).
+struct PathQueue {
+  float *ox, *oy, *oz;
+  float *dx, *dy, *dz;
+  float *ax, *ay, *az;      // attenuation (cpu.rs:254)
+  float *lx, *ly, *lz;      // light (cpu.rs:253)
+  uint32_t *tb;             // ticket << 8 | bounce
+  uint32_t *pidx;           // path index inside its ticket: s_local * W*H + pixel
+};
+
+// Per-iteration control block in device memory (ping-pong by parity).
+struct QueueCtl {
+  uint32_t count[2];        // paths in queue[p] at the start of an iteration
+  uint32_t pad0[2];
+  unsigned long long cursor[2];  // next global work index to generate
+  unsigned long long total;      // end of the work list
+  unsigned long long pad1;
+};
+
+// Algorithmic event counters (collected only by the counting build).
+struct DevCounters {
+  unsigned long long v[11];
+};
+enum CounterIndex {
+  C_CAMERA = 0, C_SEGMENTS, C_BOX, C_SPHERE, C_CUBE, C_SPHERE_HIT, C_CUBE_HIT,
+  C_GLASS, C_MISS, C_ACCUM, C_TONEMAP
+};
+
+}  // namespace rdh

@@ -1,0 +1,171 @@
+"""CPU checks of the C-ABI library: it loads, exports every declared symbol,
+fails cleanly without a GPU, and its host-side parts (the .rscn reader and
+writer, the BVH builder and its traversal rule) agree with the oracle."""
+import ctypes as C
+import glob
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle
+import raydar_amd as ra
+from raydar_amd import abi, scenegen
+from raydar_amd.scene import Scene, make_camera, make_object, sky_world
+from tests import raygen
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def declared_symbols():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names |= set(re.findall(r"\b(rd_[a-z0-9_]+)\s*\(", text))
+    return sorted(names)
+
+
+def test_library_exports_every_declared_symbol():
+    L = C.CDLL(ra.LIB_PATH)
+    names = declared_symbols()
+    assert len(names) >= 30
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_config_default_matches_renderer_config():
+    cfg = abi.RdConfig()
+    ra.check(ra.lib().rd_config_default(C.byref(cfg)))
+    assert (cfg.max_sample_count, cfg.max_bounces) == (1024, 12)  # renderer/mod.rs:16-23
+
+
+def test_create_without_gpu_fails_cleanly():
+    import subprocess
+    import sys
+    code = ("import raydar_amd as ra\n"
+            "try:\n    ra.HipRenderer()\n    print('CREATED')\n"
+            "except ra.RaydarError as e:\n    print('ERR', e.status, e)\n")
+    env = dict(os.environ, HIP_VISIBLE_DEVICES="-1", ROCR_VISIBLE_DEVICES="-1")
+    out = subprocess.run([sys.executable, "-c", code], cwd=ROOT, env=env, capture_output=True,
+                         text=True, timeout=120).stdout
+    assert out.startswith("ERR 4"), out
+    assert "HIP device" in out or "hip" in out.lower()
+
+
+@pytest.mark.parametrize("name", ["default.rscn", "benchmark.rscn"])
+def test_native_rscn_reader_matches_python(name):
+    path = os.path.join(GOLDEN, name)
+    a, b = Scene.load(path), ra.load_rscn_native(path)
+    for k in a.camera:
+        assert np.array_equal(np.float32(a.camera[k]), np.float32(b.camera[k])), k
+    assert a.objects.tobytes() == b.objects.tobytes()
+    assert {k: np.float32(v).tolist() for k, v in a.world.items()} == \
+        {k: np.float32(v).tolist() for k, v in b.world.items()}
+
+
+def test_benchmark_rscn_contents():
+    sc = Scene.load(os.path.join(GOLDEN, "benchmark.rscn"))
+    kinds = sc.objects["kind"]
+    assert (sc.width, sc.height) == (3840, 2160)
+    assert (kinds == abi.RD_SPHERE).sum() == 70 and (kinds == abi.RD_CUBE).sum() == 113
+
+
+@pytest.mark.parametrize("name", ["default.rscn", "benchmark.rscn"])
+def test_rscn_round_trip(tmp_path, name):
+    src = Scene.load(os.path.join(GOLDEN, name))
+    p1 = str(tmp_path / "native.rscn")
+    ra.save_rscn_native(src, p1)
+    back = ra.load_rscn_native(p1)
+    assert back.objects.tobytes() == src.objects.tobytes()
+    for k in src.camera:
+        assert np.array_equal(np.float32(src.camera[k]), np.float32(back.camera[k])), k
+    p2 = str(tmp_path / "python.rscn")
+    src.save(p2)
+    assert Scene.load(p2).objects.tobytes() == src.objects.tobytes()
+    assert ra.load_rscn_native(p2).objects.tobytes() == src.objects.tobytes()
+
+
+@pytest.mark.parametrize("text,status", [
+    ("{", abi.RD_ERR_PARSE),
+    ('{"camera": {}}', abi.RD_ERR_PARSE),
+    ("[1, 2]", abi.RD_ERR_PARSE),
+    ('{"camera": 1, "world": "Transparent", "objects": []} x', abi.RD_ERR_PARSE),
+])
+def test_rscn_parse_errors(text, status):
+    L = ra.lib()
+    p = C.POINTER(abi.RdScene)()
+    st = L.rd_scene_parse_rscn(text.encode(), len(text), C.byref(p))
+    assert st == status
+    assert ra.last_error()
+
+
+def test_rscn_missing_file():
+    p = C.POINTER(abi.RdScene)()
+    assert ra.lib().rd_scene_load_rscn(b"/nonexistent/x.rscn", C.byref(p)) == abi.RD_ERR_IO
+
+
+def test_transparent_world_parses(tmp_path):
+    sc = scenegen.plain_small(0)
+    sc.world = dict(sc.world, kind=abi.RD_WORLD_TRANSPARENT)
+    p = str(tmp_path / "t.rscn")
+    sc.save(p)
+    assert ra.load_rscn_native(p).world["kind"] == abi.RD_WORLD_TRANSPARENT
+
+
+# ------------------------------------------------------------------ BVH ----
+def _random_scene(seed, n):
+    rng = np.random.default_rng(seed)
+    objs = [make_object(int(rng.integers(0, 2)), rng.uniform(-20, 20, 3), float(rng.uniform(0.01, 3)))
+            for _ in range(n)]
+    return Scene(make_camera((0, 5, 40), (0, 0, 0), (0, 1, 0), 64, 64, fov_deg=50), sky_world(),
+                 np.array(objs))
+
+
+@pytest.mark.parametrize("leaf", [1, 2, 4, 8])
+@pytest.mark.parametrize("n", [1, 2, 7, 100, 2048])
+def test_bvh_invariants(leaf, n):
+    st, nodes, depth, err = ra.host_bvh_info(_random_scene(n, n), leaf)
+    assert st == abi.RD_OK, err
+    assert nodes >= 1 and depth <= 38
+
+
+def test_bvh_degenerate_inputs():
+    same = Scene(make_camera((0, 0, 9), (0, 0, 0), (0, 1, 0), 8, 8, fov_deg=40), sky_world(),
+                 np.array([make_object(abi.RD_SPHERE, (1, 2, 3), 0.5)] * 300))
+    st, _, depth, err = ra.host_bvh_info(same, 1)
+    assert st == abi.RD_OK, err
+    ids, _ = ra.host_trace(same, np.array([[1, 2, 9, 0, 0, -1]], np.float32), 1)
+    assert ids[0] == 0  # 300 coincident spheres: the lowest index wins
+    empty = Scene(same.camera, sky_world(), np.zeros(0, abi.OBJECT_DTYPE))
+    ids, _ = ra.host_trace(empty, np.array([[0, 0, 9, 0, 0, -1]], np.float32))
+    assert ids[0] == -1
+
+
+@pytest.mark.parametrize("scene_name", ["plain_small", "plain_1080p", "clustered_1024", "benchmark"])
+@pytest.mark.parametrize("leaf", [1, 4, 8])
+def test_bvh_traversal_matches_linear_scan(scene_name, leaf):
+    """The BVH culling rule (padded boxes, cull_limit) never drops the scan's
+    answer, phantom hits of the FP32 sphere test included."""
+    sc = {"plain_small": lambda: scenegen.plain_small(0),
+          "plain_1080p": lambda: scenegen.plain_1080p(1),
+          "clustered_1024": lambda: scenegen.clustered(2, 1024, 320, 180),
+          "benchmark": lambda: Scene.load(os.path.join(GOLDEN, "benchmark.rscn"))}[scene_name]()
+    rays = raygen.mixed(sc, 2500, seed=leaf)
+    ids_o, ts_o, _, _ = oracle.trace(sc, rays)
+    ids_h, ts_h = ra.host_trace(sc, rays, leaf)
+    np.testing.assert_array_equal(ids_h, ids_o)
+    hit = ids_o >= 0
+    np.testing.assert_array_equal(ts_h[hit].view(np.uint32), ts_o[hit].view(np.uint32))
+
+
+def test_nonfinite_rays_use_the_scan():
+    sc = scenegen.plain_1080p(1)
+    rays = raygen.mixed(sc, 50, seed=9)
+    rays[::3, 3] = np.nan
+    rays[1::3, 4] = np.inf
+    ids_o, ts_o, _, _ = oracle.trace(sc, rays)
+    ids_h, ts_h = ra.host_trace(sc, rays, 4)
+    np.testing.assert_array_equal(ids_h, ids_o)

@@ -1,0 +1,169 @@
+"""Whole-frame parity of the HIP renderer through the C ABI.
+
+Images are compared with the oracle's FP32 render on the same draws (the
+kernels reproduce its arithmetic, so accumulators agree to 1e-5 relative and
+normally bit for bit), against closed forms, and, for the image-error check
+of the path's contract, against an FP64 high-spp render: the GPU image's
+per-pixel RMSE to that truth must be within 1% of the CPU FP32 renderer's at
+equal spp (tests/golden/plain_small_truth.npz, made by make_truth.py).
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+import raydar_amd as ra
+from raydar_amd import abi, scenegen
+from raydar_amd.scene import Scene, make_camera, make_object, sky_world, solid_world
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _close(a, b, rel=1e-5):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return np.abs(a - b) <= rel * np.maximum(np.maximum(np.abs(a), np.abs(b)), 1.0)
+
+
+def _gpu_acc(scene, spp, bounces, seed, **opts):
+    with ra.HipRenderer(max_sample_count=spp, max_bounces=bounces, seed=seed, **opts) as r:
+        img = r.render_frame(scene)
+        acc = r.accumulator()
+        assert r.sample_count == spp
+    return img, acc
+
+
+@pytest.mark.parametrize("opts", [dict(), dict(leaf=1, block=64, queue=4096),
+                                  dict(leaf=8, block=128, queue=65536), dict(traversal=1)])
+def test_plain_small_matches_oracle(opts):
+    sc = scenegen.plain_small(0)
+    img, acc = _gpu_acc(sc, 16, 12, 5, **opts)
+    ref = oracle.render(sc, 16, 12, 5)
+    assert np.all(_close(acc, ref))
+    exact = np.mean(acc.view(np.uint32) == ref.view(np.uint32))
+    print(f"opts={opts}: accumulator bit-exact fraction {exact:.6f}")
+    assert exact == 1.0
+    np.testing.assert_array_equal(img.reshape(-1), oracle.tonemap(ref, 16))
+
+
+def test_default_rscn_matches_oracle():
+    sc = Scene.load(os.path.join(GOLDEN, "default.rscn"))
+    img, acc = _gpu_acc(sc, 4, 12, 11)
+    ref = oracle.render(sc, 4, 12, 11)
+    assert np.all(_close(acc, ref))
+    assert np.mean(acc.view(np.uint32) == ref.view(np.uint32)) == 1.0
+
+
+def test_progressive_samples_equal_full_frame():
+    """render_sample one pass at a time (cpu.rs:142-158) == render_frame."""
+    sc = scenegen.plain_small(0)
+    with ra.HipRenderer(max_sample_count=6, max_bounces=8, seed=3) as r:
+        full = r.render_frame(sc)
+        acc_full = r.accumulator()
+    with ra.HipRenderer(max_sample_count=6, max_bounces=8, seed=3) as r:
+        r.new_frame(sc)
+        r.set_samples_per_pass(4)
+        frames = []
+        while True:
+            f = r.render_sample(sc)
+            if f is None:
+                break
+            frames.append(f)
+        assert len(frames) == 2 and r.sample_count == 6
+        np.testing.assert_array_equal(r.accumulator(), acc_full)
+        np.testing.assert_array_equal(frames[-1], full)
+        prof = r.profiler()
+        assert prof["frame_ns"] is not None and prof["render_ns"] is not None
+
+
+def test_rmse_against_fp64_truth():
+    """North-star image check at equal spp, averaged over 4 seeds."""
+    z = np.load(os.path.join(GOLDEN, "plain_small_truth.npz"))
+    sc = scenegen.plain_small(0)
+    pix, truth, spp = z["pixels"], z["truth"], int(z["test_spp"])
+    g_err, c_err = [], []
+    for seed in range(4):
+        _, acc = _gpu_acc(sc, spp, 12, 100 + seed)
+        g = acc[pix, :3] / spp
+        c = oracle.render(sc, spp, 12, 100 + seed, pixels=pix)[:, :3] / spp
+        g_err.append(np.sqrt(np.mean((g - truth) ** 2)))
+        c_err.append(np.sqrt(np.mean((c - truth) ** 2)))
+    ratio = np.mean(g_err) / np.mean(c_err)
+    print(f"RMSE gpu {np.mean(g_err):.6f} cpu {np.mean(c_err):.6f} ratio {ratio:.6f}")
+    assert ratio <= 1.01
+
+
+def test_empty_scene_is_sky_closed_form():
+    cam = make_camera((0, 1, 5), (0, 0, 0), (0, 1, 0), 64, 48, fov_deg=60)
+    sc = Scene(cam, sky_world((0.2, 0.4, 0.9), (1.0, 0.9, 0.8)), np.zeros(0, abi.OBJECT_DTYPE))
+    _, acc = _gpu_acc(sc, 3, 12, 0)
+    rays = oracle.camera_rays(sc, np.arange(64 * 48, dtype=np.uint32))
+    sky = oracle.world(sc, rays[:, 3:])
+    assert np.all(_close(acc[:, :3] / 3, sky, 1e-6))
+    assert np.all(acc[:, 3] == 3)
+
+
+def test_single_object_solid_world_closed_form():
+    """One convex non-transmissive object under SolidColor(L): hit = albedo*L, miss = L."""
+    cam = make_camera((0, 0, 6), (0, 0, 0), (0, 1, 0), 64, 64, fov_deg=40)
+    L = (0.7, 0.5, 0.3)
+    alb = (0.25, 0.5, 0.75)
+    for kind in (abi.RD_SPHERE, abi.RD_CUBE):
+        sc = Scene(cam, solid_world(L), np.array([make_object(kind, (0, 0, 0), 1.0, albedo=alb,
+                                                               roughness=1.0)]))
+        _, acc = _gpu_acc(sc, 4, 12, 1)
+        rays = oracle.camera_rays(sc, np.arange(64 * 64, dtype=np.uint32))
+        hit = oracle.trace(sc, rays)[0] >= 0
+        v = acc[:, :3] / 4
+        assert hit.sum() > 100
+        assert np.allclose(v[hit], np.float32(alb) * np.float32(L), rtol=1e-6)
+        assert np.allclose(v[~hit], np.float32(L), rtol=1e-6)
+
+
+def test_enclosing_emitter_closed_form():
+    """Camera inside an emissive sphere: every bounce re-hits it and adds
+    emission unattenuated (cpu.rs:332-333), so pixel = max_bounces * E."""
+    cam = make_camera((0, 0, 0.5), (0, 0, -1), (0, 1, 0), 32, 32, fov_deg=60)
+    sc = Scene(cam, sky_world(), np.array([make_object(abi.RD_SPHERE, (0, 0, 0), 10.0,
+                                                       albedo=(0.5, 0.5, 0.5), roughness=1.0,
+                                                       emission_color=(1.0, 0.5, 0.25),
+                                                       emission_strength=2.0)]))
+    for bounces in (1, 5, 12):
+        _, acc = _gpu_acc(sc, 2, bounces, 2)
+        e = np.float32([2.0, 1.0, 0.5]) * bounces
+        assert np.allclose(acc[:, :3] / 2, e, rtol=1e-6)
+
+
+def test_zero_bounces_and_zero_samples():
+    sc = scenegen.plain_small(0)
+    img, acc = _gpu_acc(sc, 2, 0, 0)
+    assert np.all(acc[:, :3] == 0) and np.all(acc[:, 3] == 2)
+    assert np.all(img[..., :3] == 0) and np.all(img[..., 3] == 255)
+    with ra.HipRenderer(max_sample_count=0, max_bounces=12) as r:
+        img = r.render_frame(sc)
+        assert np.all(img == 0)  # acc/0 = NaN -> 0 (cpu.rs:224)
+        assert r.render_sample(sc) is None
+
+
+def test_transparent_world_is_an_error():
+    sc = scenegen.plain_small(0)
+    sc.world = dict(sc.world, kind=abi.RD_WORLD_TRANSPARENT)
+    with ra.HipRenderer(max_sample_count=1) as r:
+        with pytest.raises(ra.RaydarError) as e:
+            r.render_frame(sc)
+        assert e.value.status == abi.RD_ERR_UNSUPPORTED
+
+
+def test_resolution_change_reallocates():
+    a = scenegen.plain_small(0)
+    b = scenegen.plain_small(0)
+    b.camera = make_camera((0.0, 2.0, 8.0), (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), 96, 64, fov_deg=40.0)
+    with ra.HipRenderer(max_sample_count=3, max_bounces=4) as r:
+        r.new_frame(a)
+        assert r.render_sample(a).shape == (256, 256, 4)
+        out = r.render_sample(b)
+        assert out.shape == (64, 96, 4)
+        assert r.sample_count == 2
