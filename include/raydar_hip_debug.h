@@ -56,6 +56,14 @@ rd_status rd_debug_host_bvh_info(const rd_scene *scene, int leaf_size, uint32_t 
 rd_status rd_debug_host_trace(const rd_scene *scene, int leaf_size, const float *rays,
                               uint32_t n, int32_t *ids, float *ts);
 
+/* Host builds of the kernels' per-ray functions (the same source compiled
+ * for the CPU), with the layouts of rd_debug_scatter / rd_debug_camera_rays,
+ * so the CPU test-suite can check their logic against the oracle. */
+rd_status rd_debug_host_scatter(const rd_scene *scene, const float *in, const uint32_t *bits,
+                                uint32_t n, float *out);
+rd_status rd_debug_host_camera_rays(const rd_scene *scene, const uint32_t *pixels, uint32_t n,
+                                    float *out);
+
 #ifdef __cplusplus
 }
 #endif

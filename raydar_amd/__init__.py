@@ -15,7 +15,7 @@ from . import abi
 from .scene import Scene  # noqa: F401
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libraydar_hip.so")
+LIB_PATH = os.environ.get("RAYDAR_HIP_LIB") or os.path.join(_HERE, "libraydar_hip.so")
 _lib = None
 
 
@@ -73,6 +73,8 @@ def lib():
                                              C.c_char_p, C.c_size_t]),
         "rd_debug_host_trace": (C.c_int, [sp, C.c_int, C.c_void_p, C.c_uint32, C.c_void_p,
                                           C.c_void_p]),
+        "rd_debug_host_scatter": (C.c_int, [sp, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),
+        "rd_debug_host_camera_rays": (C.c_int, [sp, C.c_void_p, C.c_uint32, C.c_void_p]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -294,3 +296,21 @@ def host_trace(scene, rays, leaf=4):
     s = scene.as_rd()
     check(lib().rd_debug_host_trace(C.byref(s), leaf, _ptr(rays), len(rays), _ptr(ids), _ptr(ts)))
     return ids, ts
+
+
+def host_scatter(scene, packed, bits):
+    """The kernels' scatter compiled for the host (same layout as debug_scatter)."""
+    packed = np.ascontiguousarray(packed, np.float32).reshape(-1, 16)
+    bits = np.ascontiguousarray(bits, np.uint32).reshape(-1, 12)
+    out = np.zeros_like(packed)
+    s = scene.as_rd()
+    check(lib().rd_debug_host_scatter(C.byref(s), _ptr(packed), _ptr(bits), len(packed), _ptr(out)))
+    return out
+
+
+def host_camera_rays(scene, pixels):
+    pixels = np.ascontiguousarray(pixels, np.uint32)
+    out = np.zeros((len(pixels), 6), np.float32)
+    s = scene.as_rd()
+    check(lib().rd_debug_host_camera_rays(C.byref(s), _ptr(pixels), len(pixels), _ptr(out)))
+    return out

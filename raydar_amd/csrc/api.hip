@@ -21,6 +21,7 @@
 #include "bvh.hpp"
 #include "engine.hpp"
 #include "kernels.hpp"
+#include "pathmath.hpp"
 
 namespace rdh {
 
@@ -630,6 +631,52 @@ rd_status rd_debug_host_bvh_info(const rd_scene *scene, int leaf, uint32_t *n_no
     std::snprintf(err, cap, "%s", e.c_str());
   }
   return ok ? RD_OK : RD_ERR_INVALID_ARGUMENT;
+}
+
+rd_status rd_debug_host_scatter(const rd_scene *scene, const float *in, const uint32_t *bits,
+                                uint32_t n, float *out) {
+  if (!scene || !in || !bits || !out) return RD_ERR_INVALID_ARGUMENT;
+  rdh::BuiltScene b;
+  std::string e;
+  if (!rdh::build_scene(*scene, 4, &b, &e)) {
+    rdh::set_global_error(e);
+    return RD_ERR_INVALID_ARGUMENT;
+  }
+  for (uint32_t i = 0; i < n; ++i) {
+    const float *q = in + 16ull * i;
+    uint32_t id;
+    std::memcpy(&id, q + 7, 4);
+    if (id >= scene->object_count) return RD_ERR_INVALID_ARGUMENT;
+    const rdh::F3 o = rdh::f3(q[0], q[1], q[2]), d = rdh::f3(q[3], q[4], q[5]);
+    const rdh::DevObject &ob = b.objects[id];
+    const rdh::HitRec h = rdh::closest_hit(o, d, q[6], ob);
+    rdh::FixedDraws dr{bits + 12ull * i};
+    const rdh::ScatterOut so = rdh::scatter(d, h, ob, dr, rdh::f3(q[8], q[9], q[10]),
+                                            rdh::f3(q[11], q[12], q[13]));
+    float *w = out + 16ull * i;
+    const float v[16] = {so.o.x, so.o.y, so.o.z, so.d.x, so.d.y, so.d.z, so.atten.x, so.atten.y,
+                         so.atten.z, so.light.x, so.light.y, so.light.z,
+                         so.transmission ? 1.0f : 0.0f, h.n.x, h.n.y, h.n.z};
+    std::memcpy(w, v, sizeof(v));
+  }
+  return RD_OK;
+}
+
+rd_status rd_debug_host_camera_rays(const rd_scene *scene, const uint32_t *pixels, uint32_t n,
+                                    float *out) {
+  if (!scene || !pixels || !out) return RD_ERR_INVALID_ARGUMENT;
+  rdh::DeviceScene ds;
+  rdh::DevJob j;
+  rdh::fill_job(*scene, ds, nullptr, 0, 0, &j);
+  const uint64_t np = (uint64_t)j.width * j.height;
+  for (uint32_t i = 0; i < n; ++i) {
+    if (pixels[i] >= np) return RD_ERR_INVALID_ARGUMENT;
+    rdh::F3 o, d;
+    rdh::camera_ray(j, pixels[i] % j.width, pixels[i] / j.width, &o, &d);
+    const float v[6] = {o.x, o.y, o.z, d.x, d.y, d.z};
+    std::memcpy(out + 6ull * i, v, sizeof(v));
+  }
+  return RD_OK;
 }
 
 rd_status rd_debug_host_trace(const rd_scene *scene, int leaf, const float *rays, uint32_t n,

@@ -1,0 +1,888 @@
+// Streaming wavefront engine behind the batched renderer and the render
+// server (include/raydar_hip_batch.h).
+//
+// Every admitted job owns device-resident state (BVH, primitives, materials,
+// an RGBA32F accumulator, an RGBA8 image) and a slot in a fixed-capacity
+// device job table.  Its samples are cut into passes; each pass becomes a
+// ticket appended to one global work list.  The wavefront kernels consume the
+// work list through a cursor, so paths of many jobs share the same queues:
+// new camera paths fill whatever room the survivors of earlier jobs leave.
+// The host loop ("pump") admits jobs under the HBM budget, appends tickets
+// while the radiance ring has room, replays a captured hipGraph of two
+// iterations, then reads back the per-ticket finished-path counts and
+// accumulates every finished pass in pass order.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/raydar_hip_batch.h"
+#include "bvh.hpp"
+#include "engine.hpp"
+#include "kernels.hpp"
+
+namespace rdh {
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+constexpr uint32_t kJobSlots = 4096;        // device job table entries
+constexpr uint32_t kTicketCap = 1u << 20;   // device ticket table entries
+
+int64_t ns_between(Clock::time_point a, Clock::time_point b) {
+  return std::chrono::duration_cast<std::chrono::nanoseconds>(b - a).count();
+}
+
+struct SJob {
+  uint64_t id = 0;
+  uint32_t slot = 0;
+  rd_scene scene{};                // header copy; objects point into `objects`
+  std::vector<rd_object> objects;
+  BuiltScene built;
+  DeviceScene dev;
+  float *acc = nullptr;
+  uchar4 *rgba_d = nullptr;
+  uint32_t W = 0, H = 0, spp = 0, bounces = 0, pass_spp = 0;
+  uint64_t seed = 0;
+  uint32_t passes_total = 0, passes_issued = 0, passes_done = 0;
+  std::deque<uint32_t> tickets;    // issued, not yet accumulated (in pass order)
+  uint8_t *out_rgba = nullptr;
+  float *out_acc = nullptr;
+  uint8_t *progress = nullptr;
+  uint64_t bytes = 0;
+  int state = 0;                   // 0 queued, 1 resident, 2 done, 3 failed
+  int32_t status = RD_OK;
+  Clock::time_point t_submit, t_admit, t_done;
+};
+
+struct LAlloc {
+  uint64_t off, size;
+  uint32_t ticket;
+  bool freed;
+};
+
+}  // namespace
+
+class Streamer {
+ public:
+  Engine eng;
+  std::string err;
+  uint64_t budget = 200ull << 30;
+  int use_graph = 1;
+  int iters_per_pump = 2;
+  uint32_t default_pass = 0;
+  // statistics
+  uint64_t paths_done = 0, graph_launches = 0, pumps = 0;
+  uint64_t hbm_in_use = 0, hbm_peak = 0;
+  uint32_t max_resident = 0;
+
+  ~Streamer() { teardown(); }
+
+  bool init() {
+    if (!eng.init(&err)) return false;
+    return true;
+  }
+
+  // Fixed-capacity tables and engine buffers (pointers stay stable so the
+  // captured graph's kernel arguments never change).
+  bool ensure_ready() {
+    if (ready_) return true;
+    if (!eng.ensure_buffers(&err)) return false;
+    if (!eng.ensure_tickets(kTicketCap, &err)) return false;
+    if (!eng.check(hipMalloc(&d_jobs_, sizeof(DevJob) * kJobSlots), "hipMalloc(job table)", &err))
+      return false;
+    if (!eng.check(hipMemsetAsync(eng.d_ticket_done, 0, 8ull * kTicketCap, eng.stream), "hipMemset", &err))
+      return false;
+    if (!eng.check(launch_set_ctl(eng.ctl, 0, 0, eng.stream), "set ctl", &err)) return false;
+    if (!eng.check(hipHostMalloc(&done_host_, 8ull * kTicketCap, hipHostMallocDefault), "hipHostMalloc", &err))
+      return false;
+    for (uint32_t s = kJobSlots; s-- > 0;) free_slots_.push_back(s);
+    h_jobs_.assign(kJobSlots, DevJob{});
+    tickets_.reserve(kTicketCap);  // never reallocates: async uploads read from it
+    fixed_bytes_ = engine_bytes();
+    ready_ = true;
+    return eng.check(hipStreamSynchronize(eng.stream), "init", &err);
+  }
+
+  uint64_t engine_bytes() const {
+    return (uint64_t)kSegments * eng.seg_cap * (2 * 56 + 8) + eng.l_alloc * 16 + kTicketCap * 40ull +
+           sizeof(DevJob) * (uint64_t)kJobSlots;
+  }
+
+  // Validates and prepares a job on the caller's thread (scene copy + BVH).
+  std::unique_ptr<SJob> make_job(const rd_job_desc &d, int leaf, std::string *e) {
+    if (!d.scene) { *e = "job scene is NULL"; return nullptr; }
+    const rd_scene &s = *d.scene;
+    if (s.camera.resolution_x == 0 || s.camera.resolution_y == 0) { *e = "zero-size image"; return nullptr; }
+    if (s.world.kind == RD_WORLD_TRANSPARENT) {
+      *e = "transparent world is not supported (raydar: todo!(\"transparent world support\"))";
+      return nullptr;
+    }
+    if (d.samples == 0 || d.max_bounces == 0) { *e = "samples and max_bounces must be >= 1"; return nullptr; }
+    if (d.max_bounces > 255) { *e = "max_bounces above 255"; return nullptr; }
+    auto j = std::make_unique<SJob>();
+    j->scene = s;
+    j->objects.assign(s.objects, s.objects + s.object_count);
+    j->scene.objects = j->objects.data();
+    if (!build_scene(j->scene, leaf, &j->built, e) || !check_bvh(j->built, e)) return nullptr;
+    j->W = s.camera.resolution_x;
+    j->H = s.camera.resolution_y;
+    j->spp = d.samples;
+    j->bounces = d.max_bounces;
+    j->seed = d.seed;
+    const uint64_t npix = (uint64_t)j->W * j->H;
+    uint32_t pass = d.pass_samples ? d.pass_samples : (default_pass ? default_pass : d.samples);
+    pass = std::min(pass, d.samples);
+    const uint64_t fit = std::max<uint64_t>(1, eng.opt.l_capacity / 2 / npix);
+    pass = (uint32_t)std::min<uint64_t>({(uint64_t)pass, fit, 0xffffffffull / npix});
+    if (pass == 0) { *e = "image too large for the radiance arena"; return nullptr; }
+    j->pass_spp = pass;
+    j->passes_total = (d.samples + pass - 1) / pass;
+    j->out_rgba = d.rgba;
+    j->out_acc = d.accumulator;
+    j->progress = d.progress_rgba;
+    j->bytes = j->built.nodes.size() * sizeof(DevNode) + j->built.prims.size() * sizeof(DevPrim) +
+               j->built.objects.size() * sizeof(DevObject) + npix * 20;
+    j->t_submit = Clock::now();
+    return j;
+  }
+
+  bool fits_alone(const SJob &j) const { return j.bytes + fixed_bytes_ <= budget; }
+
+  void enqueue(std::unique_ptr<SJob> j) {
+    queued_.push_back(j.get());
+    owned_[j->id] = std::move(j);
+  }
+
+  bool idle() const { return queued_.empty() && resident_.empty(); }
+  uint32_t n_resident() const { return (uint32_t)resident_.size(); }
+  uint32_t n_queued() const { return (uint32_t)queued_.size(); }
+  SJob *job(uint64_t id) {
+    auto it = owned_.find(id);
+    return it == owned_.end() ? nullptr : it->second.get();
+  }
+  void forget(uint64_t id) { owned_.erase(id); }
+
+  // One round of the host loop.  Returns false on a device error.
+  bool pump() {
+    if (!ensure_ready()) return false;
+    ++pumps;
+    // Launch-shape options may have changed since the queues were sized;
+    // resize only while no path is in flight.
+    if (last_count_ == 0 && ctl_total_ == last_cursor_ && !eng.ensure_buffers(&err)) return false;
+    if (!admit()) return false;
+    if (!issue()) return false;
+    const bool work = ctl_total_ > last_cursor_ || last_count_ > 0;
+    if (work) {
+      if (!run_iterations()) return false;
+    }
+    return harvest();
+  }
+
+  uint64_t iterations() const { return eng.iterations; }
+
+ private:
+  bool ready_ = false;
+  DevJob *d_jobs_ = nullptr;
+  unsigned long long *done_host_ = nullptr;
+  std::vector<uint32_t> free_slots_;
+  std::deque<SJob *> queued_;
+  std::vector<SJob *> resident_;
+  std::map<uint64_t, std::unique_ptr<SJob>> owned_;
+  std::vector<DevTicket> tickets_;
+  std::vector<DevJob> h_jobs_;
+  std::vector<SJob *> ticket_job_;
+  std::vector<uint8_t> ticket_fin_;
+  uint32_t uploaded_ = 0, ticket_lo_ = 0;
+  uint64_t ctl_total_ = 0, last_cursor_ = 0;
+  uint32_t last_count_ = 0;
+  uint32_t stalls_ = 0;
+  int parity_ = 0;
+  std::deque<LAlloc> ring_;
+  uint64_t fixed_bytes_ = 0;
+  hipGraphExec_t graph_ = nullptr;
+  hipGraph_t graph_def_ = nullptr;
+  int graph_block_ = 0, graph_grid_ = 0, graph_brute_ = 0;
+
+  void teardown() {
+    if (graph_) (void)hipGraphExecDestroy(graph_);
+    if (graph_def_) (void)hipGraphDestroy(graph_def_);
+    graph_ = nullptr;
+    graph_def_ = nullptr;
+    for (auto &kv : owned_) release_job(kv.second.get());
+    if (d_jobs_) (void)hipFree(d_jobs_);
+    d_jobs_ = nullptr;
+    if (done_host_) (void)hipHostFree(done_host_);
+    done_host_ = nullptr;
+  }
+
+  void release_job(SJob *j) {
+    if (j->state != 1 && !j->acc && !j->dev.nodes && !j->dev.prims) return;
+    eng.free_scene(&j->dev);
+    if (j->acc) (void)hipFree(j->acc);
+    if (j->rgba_d) (void)hipFree(j->rgba_d);
+    j->acc = nullptr;
+    j->rgba_d = nullptr;
+  }
+
+  bool admit() {
+    while (!queued_.empty()) {
+      SJob *j = queued_.front();
+      if (free_slots_.empty()) break;
+      if (hbm_in_use + j->bytes + fixed_bytes_ > budget && !resident_.empty()) break;
+      queued_.pop_front();
+      j->slot = free_slots_.back();
+      free_slots_.pop_back();
+      const uint64_t npix = (uint64_t)j->W * j->H;
+      if (!eng.upload_scene(j->built, &j->dev, &err)) return false;
+      if (!eng.check(hipMalloc(&j->acc, 16 * npix), "hipMalloc(accumulator)", &err)) return false;
+      if (!eng.check(hipMalloc(&j->rgba_d, 4 * npix), "hipMalloc(image)", &err)) return false;
+      if (!eng.check(launch_zero_acc(j->acc, 4 * npix, eng.stream), "zero accumulator", &err)) return false;
+      // the host mirror entry outlives the asynchronous copy
+      DevJob &dj = h_jobs_[j->slot];
+      fill_job(j->scene, j->dev, j->acc, j->bounces, j->seed, &dj);
+      if (!eng.check(hipMemcpyAsync(d_jobs_ + j->slot, &dj, sizeof(DevJob), hipMemcpyHostToDevice,
+                                    eng.stream),
+                     "upload job", &err))
+        return false;
+      if (j->built.nodes.size() > kNarrowStackNodes) wide_ = true;
+      j->built = BuiltScene();  // host copy no longer needed
+      j->state = 1;
+      j->t_admit = Clock::now();
+      resident_.push_back(j);
+      hbm_in_use += j->bytes;
+      hbm_peak = std::max(hbm_peak, hbm_in_use);
+      max_resident = std::max<uint32_t>(max_resident, (uint32_t)resident_.size());
+    }
+    return true;
+  }
+
+  // Ring allocation of `n` radiance slots; false when the ring is full.
+  bool ring_alloc(uint64_t n, uint64_t *off) {
+    const uint64_t cap = eng.l_alloc;
+    if (n > cap) return false;
+    if (ring_.empty()) {
+      *off = 0;
+      return true;
+    }
+    const uint64_t tail = ring_.front().off;
+    const uint64_t head = ring_.back().off + ring_.back().size;
+    if (head >= tail) {  // free space [head, cap) and [0, tail)
+      if (cap - head >= n) { *off = head; return true; }
+      if (tail >= n) { *off = 0; return true; }
+      return false;
+    }
+    if (tail - head >= n) { *off = head; return true; }
+    return false;
+  }
+
+  bool issue() {
+    // Keep about two queue loads of work ahead of the cursor.
+    const uint64_t ahead = 2ull * eng.q_alloc;
+    bool added = false;
+    for (SJob *j : resident_) {
+      while (j->passes_issued < j->passes_total && ctl_total_ - last_cursor_ < ahead) {
+        if (tickets_.size() >= kTicketCap) break;
+        const uint64_t npix = (uint64_t)j->W * j->H;
+        const uint32_t s0 = j->passes_issued * j->pass_spp;
+        const uint32_t spp = std::min(j->pass_spp, j->spp - s0);
+        uint64_t off;
+        if (!ring_alloc(spp * npix, &off)) break;
+        DevTicket t{};
+        t.work_begin = ctl_total_;
+        t.l_offset = off;
+        t.job = j->slot;
+        t.sample_base = s0;
+        t.spp = spp;
+        t.paths = (uint32_t)(spp * npix);
+        const uint32_t ti = (uint32_t)tickets_.size();
+        tickets_.push_back(t);
+        ticket_job_.push_back(j);
+        ticket_fin_.push_back(0);
+        ring_.push_back(LAlloc{off, (uint64_t)t.paths, ti, false});
+        j->tickets.push_back(ti);
+        j->passes_issued++;
+        ctl_total_ += t.paths;
+        added = true;
+      }
+    }
+    if (!added) return true;
+    const uint32_t n = (uint32_t)tickets_.size();
+    if (!eng.check(hipMemcpyAsync(eng.d_tickets + uploaded_, tickets_.data() + uploaded_,
+                                  sizeof(DevTicket) * (n - uploaded_), hipMemcpyHostToDevice,
+                                  eng.stream),
+                   "upload tickets", &err))
+      return false;
+    uploaded_ = n;
+    return eng.check(launch_extend_work(eng.ctl, ctl_total_, n, ticket_lo_, eng.stream),
+                     "append work", &err);
+  }
+
+  bool wide_ = false;
+
+  IterArgs args() const {
+    IterArgs a = eng.iter_args(d_jobs_);
+    a.single_job = 0;
+    a.track_done = 1;
+    a.wide_stack = wide_ ? 1 : 0;
+    return a;
+  }
+
+  bool capture() {
+    if (graph_) {
+      (void)hipGraphExecDestroy(graph_);
+      (void)hipGraphDestroy(graph_def_);
+      graph_ = nullptr;
+      graph_def_ = nullptr;
+    }
+    const IterArgs a = args();
+    if (!eng.check(hipStreamBeginCapture(eng.stream, hipStreamCaptureModeThreadLocal), "begin capture", &err))
+      return false;
+    hipError_t e = hipSuccess;
+    for (int p = 0; p < 2 && e == hipSuccess; ++p) {
+      e = launch_generate(a, p, eng.shape(0), false, eng.stream);
+      if (e == hipSuccess) e = launch_extend(a, p, eng.shape(1), false, eng.stream);
+      if (e == hipSuccess) e = launch_shade(a, p, eng.shape(2), false, eng.stream);
+    }
+    hipError_t e2 = hipStreamEndCapture(eng.stream, &graph_def_);
+    if (!eng.check(e, "capture", &err) || !eng.check(e2, "end capture", &err)) return false;
+    if (!eng.check(hipGraphInstantiate(&graph_, graph_def_, nullptr, nullptr, 0), "graph instantiate", &err))
+      return false;
+    graph_block_ = eng.opt.block;
+    graph_grid_ = eng.opt.grid;
+    graph_brute_ = eng.opt.brute_force;
+    graph_wide_ = wide_;
+    graph_gen_ = eng.generation;
+    return true;
+  }
+  bool graph_wide_ = false;
+  uint64_t graph_gen_ = 0;
+
+  bool run_iterations() {
+    const int n = std::max(2, iters_per_pump & ~1);
+    const bool direct = !use_graph || eng.opt.count || eng.opt.kernel_timing;
+    if (direct) {
+      const IterArgs a = args();
+      for (int i = 0; i < n; ++i) {
+        if (!eng.launch_iteration(a, parity_, &err)) return false;
+        parity_ ^= 1;
+      }
+      return true;
+    }
+    if (!graph_ || graph_block_ != eng.opt.block || graph_grid_ != eng.opt.grid ||
+        graph_brute_ != eng.opt.brute_force || graph_wide_ != wide_ ||
+        graph_gen_ != eng.generation) {
+      if (!capture()) return false;
+    }
+    for (int i = 0; i < n; i += 2) {
+      if (!eng.check(hipGraphLaunch(graph_, eng.stream), "graph launch", &err)) return false;
+      ++graph_launches;
+      eng.iterations += 2;
+    }
+    return true;
+  }
+
+  bool harvest() {
+    const uint32_t n = uploaded_;
+    if (!eng.check(hipMemcpyAsync(eng.ctl_host, eng.ctl, sizeof(QueueCtl), hipMemcpyDeviceToHost,
+                                  eng.stream),
+                   "read queue state", &err))
+      return false;
+    if (n > ticket_lo_ &&
+        !eng.check(hipMemcpyAsync(done_host_ + ticket_lo_, eng.d_ticket_done + ticket_lo_,
+                                  8ull * (n - ticket_lo_), hipMemcpyDeviceToHost, eng.stream),
+                   "read ticket progress", &err))
+      return false;
+    if (!eng.check(hipStreamSynchronize(eng.stream), "iteration", &err)) return false;
+    if (eng.ctl_host->overflow) {
+      err = "path queue segment overflow (internal error)";
+      return false;
+    }
+    const uint32_t count = Engine::queued(*eng.ctl_host, parity_);
+    const uint64_t cursor = eng.ctl_host->cursor[parity_];
+    // every iteration either generates paths or advances every path in flight
+    // by one bounce, so the state cannot stay frozen while work is pending
+    if (count == last_count_ && cursor == last_cursor_ && (count > 0 || cursor < ctl_total_)) {
+      if (++stalls_ > 1000) {
+        err = "wavefront loop made no progress (internal error)";
+        return false;
+      }
+    } else {
+      stalls_ = 0;
+    }
+    last_count_ = count;
+    last_cursor_ = cursor;
+    for (uint32_t t = ticket_lo_; t < n; ++t)
+      if (!ticket_fin_[t] && done_host_[t] == tickets_[t].paths) ticket_fin_[t] = 1;
+    // accumulate finished passes in pass order, per job
+    std::vector<SJob *> finished;
+    for (SJob *j : resident_) {
+      bool progressed = false;
+      while (!j->tickets.empty() && ticket_fin_[j->tickets.front()] == 1) {
+        const uint32_t ti = j->tickets.front();
+        j->tickets.pop_front();
+        const DevTicket &t = tickets_[ti];
+        const int grid = (int)std::min<uint64_t>(((uint64_t)j->W * j->H + 255) / 256, 8192);
+        if (!eng.check(launch_accumulate(d_jobs_ + j->slot, t, eng.L, eng.counters, eng.opt.count != 0,
+                                         std::max(grid, 1), eng.stream),
+                       "accumulate", &err))
+          return false;
+        ticket_fin_[ti] = 2;
+        for (LAlloc &a : ring_)
+          if (a.ticket == ti) a.freed = true;
+        j->passes_done++;
+        paths_done += t.paths;
+        progressed = true;
+      }
+      if (progressed && (j->progress || j->passes_done == j->passes_total)) {
+        const uint64_t npix = (uint64_t)j->W * j->H;
+        const int grid = (int)std::min<uint64_t>((npix + 255) / 256, 8192);
+        const uint32_t samples = std::min(j->spp, j->passes_done * j->pass_spp);
+        if (!eng.check(launch_tonemap(d_jobs_ + j->slot, (uint32_t)npix, (float)samples, j->rgba_d,
+                                      eng.counters, false, std::max(grid, 1), eng.stream),
+                       "tonemap", &err))
+          return false;
+        uint8_t *dst = j->passes_done == j->passes_total ? j->out_rgba : j->progress;
+        if (j->passes_done == j->passes_total && j->progress &&
+            !eng.check(hipMemcpyAsync(j->progress, j->rgba_d, 4 * npix, hipMemcpyDeviceToHost, eng.stream),
+                       "read image", &err))
+          return false;
+        if (dst && !eng.check(hipMemcpyAsync(dst, j->rgba_d, 4 * npix, hipMemcpyDeviceToHost, eng.stream),
+                              "read image", &err))
+          return false;
+      }
+      if (j->passes_done == j->passes_total) {
+        const uint64_t npix = (uint64_t)j->W * j->H;
+        if (j->out_acc && !eng.check(hipMemcpyAsync(j->out_acc, j->acc, 16 * npix, hipMemcpyDeviceToHost, eng.stream),
+                                     "read accumulator", &err))
+          return false;
+        finished.push_back(j);
+      }
+    }
+    while (!ring_.empty() && ring_.front().freed) ring_.pop_front();
+    while (ticket_lo_ < n && ticket_fin_[ticket_lo_] == 2) ++ticket_lo_;
+    if (!finished.empty()) {
+      if (!eng.check(hipStreamSynchronize(eng.stream), "read results", &err)) return false;
+      const auto now = Clock::now();
+      for (SJob *j : finished) {
+        j->t_done = now;
+        j->state = 2;
+        release_job(j);
+        free_slots_.push_back(j->slot);
+        hbm_in_use -= j->bytes;
+        resident_.erase(std::find(resident_.begin(), resident_.end(), j));
+      }
+    }
+    return true;
+  }
+};
+
+}  // namespace rdh
+
+// ------------------------------------------------------------- batch ABI ---
+struct rd_batch {
+  rdh::Streamer st;
+  std::string last_error;
+  rd_batch_stats stats{};
+  uint64_t next_id = 1;
+};
+
+namespace {
+
+rd_status batch_fail(rd_batch *b, rd_status s, const std::string &m) {
+  b->last_error = m;
+  return s;
+}
+
+bool engine_option(rdh::Streamer &st, const std::string &k, int64_t v, std::string *err) {
+  rdh::EngineOptions &o = st.eng.opt;
+  if (k == "block") {
+    if (v != 64 && v != 128 && v != 256) { *err = "block must be 64, 128 or 256"; return false; }
+    o.block = (int)v;
+  } else if (k == "queue") {
+    if (v < 1024 || v > (1ll << 28)) { *err = "queue out of range"; return false; }
+    if (st.eng.q_alloc && (uint32_t)v != st.eng.q_alloc) { *err = "queue is fixed once rendering started"; return false; }
+    o.queue = (uint32_t)v;
+  } else if (k == "leaf") {
+    if (v < 1 || v > rdh::kMaxLeafSize) { *err = "leaf must be in 1..8"; return false; }
+    o.leaf = (int)v;
+  } else if (k == "grid") {
+    if (v < 0 || v > (1 << 20)) { *err = "grid out of range"; return false; }
+    o.grid = (int)v;
+  } else if (k == "graph") {
+    st.use_graph = v != 0;
+  } else if (k == "iters_per_pump") {
+    if (v < 2 || v > 1024) { *err = "iters_per_pump out of range"; return false; }
+    st.iters_per_pump = (int)v;
+  } else if (k == "l_capacity") {
+    if (v < (1 << 16) || v > (1ll << 32)) { *err = "l_capacity out of range"; return false; }
+    if (st.eng.l_alloc && (uint64_t)v != st.eng.l_alloc) { *err = "l_capacity is fixed once rendering started"; return false; }
+    o.l_capacity = (uint64_t)v;
+  } else if (k == "traversal") {
+    o.brute_force = v != 0;
+  } else if (k == "count") {
+    o.count = v != 0;
+  } else if (k == "kernel_timing") {
+    o.kernel_timing = v != 0;
+  } else {
+    *err = "unknown option " + k;
+    return false;
+  }
+  return true;
+}
+
+bool engine_option_get(const rdh::Streamer &st, const std::string &k, int64_t *v) {
+  const rdh::EngineOptions &o = st.eng.opt;
+  if (k == "block") *v = o.block;
+  else if (k == "queue") *v = o.queue;
+  else if (k == "leaf") *v = o.leaf;
+  else if (k == "grid") *v = o.grid;
+  else if (k == "graph") *v = st.use_graph;
+  else if (k == "iters_per_pump") *v = st.iters_per_pump;
+  else if (k == "l_capacity") *v = (int64_t)o.l_capacity;
+  else if (k == "num_cus") *v = st.eng.num_cus;
+  else return false;
+  return true;
+}
+
+rd_status run_batch(rd_batch *b, const rd_job_desc *jobs, uint32_t n) {
+  rdh::Streamer &st = b->st;
+  std::vector<uint64_t> ids;
+  const auto t0 = std::chrono::steady_clock::now();
+  const uint64_t it0 = st.iterations(), g0 = st.graph_launches, p0 = st.pumps, paths0 = st.paths_done;
+  for (uint32_t i = 0; i < n; ++i) {
+    std::string e;
+    auto j = st.make_job(jobs[i], st.eng.opt.leaf, &e);
+    if (!j) return batch_fail(b, e.find("transparent") != std::string::npos ? RD_ERR_UNSUPPORTED : RD_ERR_INVALID_ARGUMENT,
+                              "job " + std::to_string(i) + ": " + e);
+    j->id = b->next_id++;
+    ids.push_back(j->id);
+    st.enqueue(std::move(j));
+  }
+  while (!st.idle()) {
+    if (!st.pump()) return batch_fail(b, st.err.find("out of memory") != std::string::npos ? RD_ERR_OUT_OF_MEMORY : RD_ERR_DEVICE, st.err);
+  }
+  for (uint64_t id : ids) st.forget(id);
+  b->stats.elapsed_ns = rdh::ns_between(t0, std::chrono::steady_clock::now());
+  b->stats.iterations = st.iterations() - it0;
+  b->stats.graph_launches = st.graph_launches - g0;
+  b->stats.pumps = st.pumps - p0;
+  b->stats.paths = st.paths_done - paths0;
+  return RD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+rd_status rd_batch_create(rd_batch **out) {
+  if (!out) return RD_ERR_INVALID_ARGUMENT;
+  *out = nullptr;
+  auto *b = new (std::nothrow) rd_batch();
+  if (!b) return RD_ERR_OUT_OF_MEMORY;
+  if (!b->st.init()) {
+    rdh::set_global_error(b->st.err);
+    delete b;
+    return RD_ERR_DEVICE;
+  }
+  *out = b;
+  return RD_OK;
+}
+
+void rd_batch_destroy(rd_batch *b) {
+  if (!b) return;
+  (void)hipStreamSynchronize(b->st.eng.stream);
+  delete b;
+}
+
+rd_status rd_batch_set_option(rd_batch *b, const char *key, int64_t value) {
+  if (!b || !key) return RD_ERR_INVALID_ARGUMENT;
+  std::string e;
+  if (!engine_option(b->st, key, value, &e)) return batch_fail(b, RD_ERR_INVALID_ARGUMENT, e);
+  return RD_OK;
+}
+
+rd_status rd_batch_get_option(const rd_batch *b, const char *key, int64_t *value) {
+  if (!b || !key || !value) return RD_ERR_INVALID_ARGUMENT;
+  return engine_option_get(b->st, key, value) ? RD_OK : RD_ERR_INVALID_ARGUMENT;
+}
+
+rd_status rd_batch_render(rd_batch *b, const rd_job_desc *jobs, uint32_t n) {
+  if (!b || (n && !jobs)) return RD_ERR_INVALID_ARGUMENT;
+  return run_batch(b, jobs, n);
+}
+
+rd_status rd_batch_stats_get(const rd_batch *b, rd_batch_stats *out) {
+  if (!b || !out) return RD_ERR_INVALID_ARGUMENT;
+  *out = b->stats;
+  return RD_OK;
+}
+
+const char *rd_batch_last_error(const rd_batch *b) { return b ? b->last_error.c_str() : ""; }
+
+rd_status rd_batch_autotune(rd_batch *b, const rd_job_desc *jobs, uint32_t n, rd_tune_result *out) {
+  if (!b || !jobs || !n || !out) return RD_ERR_INVALID_ARGUMENT;
+  // The queue capacity and radiance arena are fixed once allocated, so the
+  // sweep runs on fresh batches and the winner is applied to `b`.
+  const int blocks[] = {64, 128, 256};
+  const uint32_t queues[] = {1u << 20, 1u << 22, 1u << 23, 1u << 24};
+  const int leaves[] = {1, 2, 4, 8};
+  double best = 1e300, dflt = 0.0;
+  rd_tune_result r{};
+  r.block = b->st.eng.opt.block;
+  r.leaf = b->st.eng.opt.leaf;
+  r.queue = b->st.eng.opt.queue;
+  for (uint32_t q : queues) {
+    rd_batch *t = nullptr;
+    rd_status s = rd_batch_create(&t);
+    if (s != RD_OK) return batch_fail(b, s, rd_last_error(nullptr));
+    std::unique_ptr<rd_batch, void (*)(rd_batch *)> guard(t, rd_batch_destroy);
+    t->st.eng.opt = b->st.eng.opt;
+    t->st.eng.opt.queue = q;
+    t->st.use_graph = b->st.use_graph;
+    for (int blk : blocks)
+      for (int lf : leaves) {
+        t->st.eng.opt.block = blk;
+        t->st.eng.opt.leaf = lf;
+        // one untimed run per launch shape compiles / warms the kernels
+        if ((s = run_batch(t, jobs, n)) != RD_OK) return batch_fail(b, s, t->last_error);
+        if ((s = run_batch(t, jobs, n)) != RD_OK) return batch_fail(b, s, t->last_error);
+        const double ms = t->stats.elapsed_ns / 1e6;
+        r.configs_tried++;
+        if (blk == 256 && lf == 4 && q == (1u << 23)) dflt = ms;
+        if (ms < best) {
+          best = ms;
+          r.block = blk;
+          r.leaf = lf;
+          r.queue = q;
+        }
+      }
+  }
+  r.best_ms = best;
+  r.default_ms = dflt;
+  b->st.eng.opt.block = r.block;
+  b->st.eng.opt.leaf = r.leaf;
+  if (!b->st.eng.q_alloc) b->st.eng.opt.queue = r.queue;
+  *out = r;
+  return RD_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------ server ABI ---
+struct rd_server {
+  rdh::Streamer st;
+  std::string last_error;
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<std::unique_ptr<rdh::SJob>> inbox;
+  std::map<uint64_t, rd_job_result> results;
+  uint64_t next_id = 1;
+  bool stop = false, failed = false;
+  std::thread worker;
+  rd_server_stats stats{};
+  std::chrono::steady_clock::time_point busy_since;
+  bool busy = false;
+};
+
+namespace {
+
+void server_fill_result(rdh::SJob *j, rd_job_result *r) {
+  r->status = j->status;
+  r->done = j->state == 2 ? 1 : 0;
+  r->passes_done = j->passes_done;
+  r->samples_done = std::min(j->spp, j->passes_done * j->pass_spp);
+  r->device_bytes = j->bytes;
+  r->queued_ns = j->state >= 1 ? rdh::ns_between(j->t_submit, j->t_admit) : -1;
+  r->latency_ns = j->state == 2 ? rdh::ns_between(j->t_submit, j->t_done) : -1;
+}
+
+void server_loop(rd_server *s) {
+  std::unique_lock<std::mutex> lk(s->mu);
+  for (;;) {
+    while (!s->inbox.empty()) {
+      auto j = std::move(s->inbox.front());
+      s->inbox.pop_front();
+      s->st.enqueue(std::move(j));
+    }
+    if (s->st.idle()) {
+      if (s->busy) {
+        s->stats.busy_ns += rdh::ns_between(s->busy_since, std::chrono::steady_clock::now());
+        s->busy = false;
+      }
+      if (s->stop) break;
+      s->cv_work.wait(lk, [s] { return s->stop || !s->inbox.empty(); });
+      continue;
+    }
+    if (!s->busy) {
+      s->busy = true;
+      s->busy_since = std::chrono::steady_clock::now();
+    }
+    lk.unlock();
+    const bool ok = s->st.pump();
+    lk.lock();
+    // publish progress of every known job
+    for (auto &kv : s->results) {
+      rdh::SJob *j = s->st.job(kv.first);
+      if (!j) continue;
+      server_fill_result(j, &kv.second);
+      if (j->state == 2) {
+        s->stats.jobs_done++;
+        s->st.forget(kv.first);
+      }
+    }
+    s->stats.paths = s->st.paths_done;
+    s->stats.iterations = s->st.iterations();
+    s->stats.hbm_in_use = s->st.hbm_in_use;
+    s->stats.hbm_peak = s->st.hbm_peak;
+    s->stats.engine_bytes = s->st.engine_bytes();
+    s->stats.max_resident = s->st.max_resident;
+    s->stats.jobs_resident = s->st.n_resident();
+    s->stats.jobs_queued = s->st.n_queued() + (uint32_t)s->inbox.size();
+    s->cv_done.notify_all();
+    if (!ok) {
+      s->failed = true;
+      s->last_error = s->st.err;
+      for (auto &kv : s->results)
+        if (!kv.second.done) {
+          kv.second.status = RD_ERR_DEVICE;
+          kv.second.done = 1;
+        }
+      s->cv_done.notify_all();
+      break;
+    }
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+rd_status rd_server_create(const rd_server_config *cfg, rd_server **out) {
+  if (!out) return RD_ERR_INVALID_ARGUMENT;
+  *out = nullptr;
+  auto *s = new (std::nothrow) rd_server();
+  if (!s) return RD_ERR_OUT_OF_MEMORY;
+  if (!s->st.init()) {
+    rdh::set_global_error(s->st.err);
+    delete s;
+    return RD_ERR_DEVICE;
+  }
+  s->st.default_pass = 8;
+  if (cfg) {
+    if (cfg->hbm_budget_bytes) s->st.budget = cfg->hbm_budget_bytes;
+    if (cfg->pass_samples) s->st.default_pass = cfg->pass_samples;
+    std::string e;
+    if ((cfg->block && !engine_option(s->st, "block", cfg->block, &e)) ||
+        (cfg->queue && !engine_option(s->st, "queue", cfg->queue, &e)) ||
+        (cfg->leaf && !engine_option(s->st, "leaf", cfg->leaf, &e))) {
+      rdh::set_global_error(e);
+      delete s;
+      return RD_ERR_INVALID_ARGUMENT;
+    }
+  }
+  if (!s->st.ensure_ready()) {
+    rdh::set_global_error(s->st.err);
+    delete s;
+    return RD_ERR_DEVICE;
+  }
+  if (s->st.engine_bytes() > s->st.budget) {
+    rdh::set_global_error("HBM budget smaller than the engine's own buffers");
+    delete s;
+    return RD_ERR_BUDGET;
+  }
+  s->stats.engine_bytes = s->st.engine_bytes();
+  s->worker = std::thread(server_loop, s);
+  *out = s;
+  return RD_OK;
+}
+
+void rd_server_destroy(rd_server *s) {
+  if (!s) return;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stop = true;
+  }
+  s->cv_work.notify_all();
+  if (s->worker.joinable()) s->worker.join();
+  delete s;
+}
+
+rd_status rd_server_submit(rd_server *s, const rd_job_desc *job, uint64_t *job_id) {
+  if (!s || !job || !job_id) return RD_ERR_INVALID_ARGUMENT;
+  std::string e;
+  int leaf;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->failed) return RD_ERR_DEVICE;
+    if (s->stop) return RD_ERR_INVALID_ARGUMENT;
+    leaf = s->st.eng.opt.leaf;
+  }
+  auto j = s->st.make_job(*job, leaf, &e);  // BVH build on the caller's thread
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (!j) {
+    s->last_error = e;
+    s->stats.jobs_rejected++;
+    return e.find("transparent") != std::string::npos ? RD_ERR_UNSUPPORTED : RD_ERR_INVALID_ARGUMENT;
+  }
+  if (!s->st.fits_alone(*j)) {
+    s->last_error = "job needs " + std::to_string(j->bytes) + " bytes of HBM, above the budget";
+    s->stats.jobs_rejected++;
+    return RD_ERR_BUDGET;
+  }
+  j->id = s->next_id++;
+  *job_id = j->id;
+  rd_job_result r{};
+  r.status = RD_OK;
+  r.queued_ns = -1;
+  r.latency_ns = -1;
+  r.device_bytes = j->bytes;
+  s->results[j->id] = r;
+  s->inbox.push_back(std::move(j));
+  s->stats.jobs_submitted++;
+  s->cv_work.notify_all();
+  return RD_OK;
+}
+
+rd_status rd_server_poll(rd_server *s, uint64_t id, rd_job_result *out) {
+  if (!s || !out) return RD_ERR_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> lk(s->mu);
+  auto it = s->results.find(id);
+  if (it == s->results.end()) return RD_ERR_NOT_FOUND;
+  *out = it->second;
+  return RD_OK;
+}
+
+rd_status rd_server_wait(rd_server *s, uint64_t id, int64_t timeout_ms, rd_job_result *out) {
+  if (!s || !out) return RD_ERR_INVALID_ARGUMENT;
+  std::unique_lock<std::mutex> lk(s->mu);
+  auto it = s->results.find(id);
+  if (it == s->results.end()) return RD_ERR_NOT_FOUND;
+  auto pred = [&] { return it->second.done != 0; };
+  if (timeout_ms < 0) s->cv_done.wait(lk, pred);
+  else s->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), pred);
+  *out = it->second;
+  return RD_OK;
+}
+
+rd_status rd_server_stats_get(rd_server *s, rd_server_stats *out) {
+  if (!s || !out) return RD_ERR_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> lk(s->mu);
+  *out = s->stats;
+  if (s->busy) out->busy_ns += rdh::ns_between(s->busy_since, std::chrono::steady_clock::now());
+  return RD_OK;
+}
+
+const char *rd_server_last_error(const rd_server *s) { return s ? s->last_error.c_str() : ""; }
+
+}  // extern "C"

@@ -65,30 +65,52 @@ void Engine::release_buffers() {
   if (ctl_host) { (void)hipHostFree(ctl_host); ctl_host = nullptr; }
 }
 
+// Slots per segment so that the shade workgroups sharing a segment can never
+// overfill it: workgroup b handles at most ceil(Q / (G*B)) tiles of B paths
+// and ceil(G / kSegments) workgroups share a segment.
+uint32_t Engine::segment_slots() const {
+  const LaunchShape sh = shape(2);
+  const uint64_t tile = (uint64_t)sh.grid * sh.block;
+  const uint64_t per_block = ((opt.queue + tile - 1) / tile) * sh.block;
+  const uint64_t blocks = (sh.grid + kSegments - 1) / kSegments;
+  const uint64_t even = (opt.queue + kSegments - 1) / kSegments;
+  return (uint32_t)std::max<uint64_t>(blocks * per_block, even);
+}
+
+uint32_t Engine::queued(const QueueCtl &c, int parity) {
+  uint32_t n = 0;
+  for (int s = 0; s < kSegments; ++s) n += c.count[parity][s * kSegStride];
+  return n;
+}
+
 bool Engine::ensure_buffers(std::string *err) {
   if (opt.queue < 1024) opt.queue = 1024;
-  if (q_alloc != opt.queue) {
+  const uint32_t need = segment_slots();
+  if (q_alloc != opt.queue || need > seg_cap) {
+    const uint64_t slots = (uint64_t)kSegments * need;
     for (int p = 0; p < 2; ++p) {
       float **f[] = {&q[p].ox, &q[p].oy, &q[p].oz, &q[p].dx, &q[p].dy, &q[p].dz,
                      &q[p].ax, &q[p].ay, &q[p].az, &q[p].lx, &q[p].ly, &q[p].lz};
       for (float **x : f) {
         if (*x) (void)hipFree(*x);
         *x = nullptr;
-        if (!check(hipMalloc(x, sizeof(float) * opt.queue), "hipMalloc(queue)", err)) return false;
+        if (!check(hipMalloc(x, sizeof(float) * slots), "hipMalloc(queue)", err)) return false;
       }
       if (q[p].tb) (void)hipFree(q[p].tb);
       if (q[p].pidx) (void)hipFree(q[p].pidx);
       q[p].tb = q[p].pidx = nullptr;
-      if (!check(hipMalloc(&q[p].tb, 4ull * opt.queue), "hipMalloc(queue)", err)) return false;
-      if (!check(hipMalloc(&q[p].pidx, 4ull * opt.queue), "hipMalloc(queue)", err)) return false;
+      if (!check(hipMalloc(&q[p].tb, 4ull * slots), "hipMalloc(queue)", err)) return false;
+      if (!check(hipMalloc(&q[p].pidx, 4ull * slots), "hipMalloc(queue)", err)) return false;
     }
     if (hit_t) (void)hipFree(hit_t);
     if (hit_id) (void)hipFree(hit_id);
     hit_t = nullptr;
     hit_id = nullptr;
-    if (!check(hipMalloc(&hit_t, 4ull * opt.queue), "hipMalloc(hits)", err)) return false;
-    if (!check(hipMalloc(&hit_id, 4ull * opt.queue), "hipMalloc(hits)", err)) return false;
+    if (!check(hipMalloc(&hit_t, 4ull * slots), "hipMalloc(hits)", err)) return false;
+    if (!check(hipMalloc(&hit_id, 4ull * slots), "hipMalloc(hits)", err)) return false;
     q_alloc = opt.queue;
+    seg_cap = need;
+    ++generation;
   }
   if (l_alloc < opt.l_capacity) {
     if (L) (void)hipFree(L);
@@ -96,6 +118,7 @@ bool Engine::ensure_buffers(std::string *err) {
     if (!check(hipMalloc(&L, sizeof(float4) * opt.l_capacity), "hipMalloc(radiance arena)", err))
       return false;
     l_alloc = opt.l_capacity;
+    ++generation;
   }
   if (!ctl && !check(hipMalloc(&ctl, sizeof(QueueCtl)), "hipMalloc(ctl)", err)) return false;
   if (!ctl_host &&
@@ -149,7 +172,7 @@ void Engine::free_scene(DeviceScene *s) {
   *s = DeviceScene();
 }
 
-IterArgs Engine::iter_args(const DevJob *d_jobs, uint32_t n_tickets) const {
+IterArgs Engine::iter_args(const DevJob *d_jobs) const {
   IterArgs a;
   a.q[0] = q[0];
   a.q[1] = q[1];
@@ -158,12 +181,15 @@ IterArgs Engine::iter_args(const DevJob *d_jobs, uint32_t n_tickets) const {
   a.ctl = ctl;
   a.jobs = d_jobs;
   a.tickets = d_tickets;
-  a.n_tickets = n_tickets;
   a.capacity = q_alloc;
+  a.seg_cap = seg_cap;
   a.L = L;
   a.ticket_done = d_ticket_done;
   a.counters = counters;
   a.brute_force = opt.brute_force;
+  a.single_job = 0;
+  a.track_done = 1;
+  a.wide_stack = 0;
   return a;
 }
 
@@ -236,8 +262,12 @@ bool Engine::run_tickets(const DevJob *d_jobs, std::vector<DevTicket> tickets,
                "upload tickets", err))
       return false;
     if (!check(hipMemsetAsync(d_ticket_done, 0, 8ull * n, stream), "hipMemset", err)) return false;
-    if (!check(launch_set_ctl(ctl, w, stream), "set ctl", err)) return false;
-    const IterArgs a = iter_args(d_jobs, n);
+    if (!check(launch_set_ctl(ctl, w, n, stream), "set ctl", err)) return false;
+    IterArgs a = iter_args(d_jobs);
+    a.single_job = host_jobs.size() == 1 ? 1 : 0;
+    a.track_done = 0;  // completion is known from the drained queue
+    for (const DevJob &j : host_jobs)
+      if (j.n_nodes > kNarrowStackNodes) a.wide_stack = 1;
     int parity = 0;
     const uint64_t cap_iters = (w / q_alloc + 2) * (uint64_t)(max_bounces + 1) + 64;
     uint64_t it = 0;
@@ -252,7 +282,11 @@ bool Engine::run_tickets(const DevJob *d_jobs, std::vector<DevTicket> tickets,
                  "read queue state", err))
         return false;
       if (!check(hipStreamSynchronize(stream), "iteration", err)) return false;
-      if (ctl_host->count[parity] == 0 && ctl_host->cursor[parity] == w) break;
+      if (ctl_host->overflow) {
+        *err = "path queue segment overflow (internal error)";
+        return false;
+      }
+      if (queued(*ctl_host, parity) == 0 && ctl_host->cursor[parity] == w) break;
       if (it > cap_iters) {
         *err = "wavefront loop made no progress (internal error)";
         return false;

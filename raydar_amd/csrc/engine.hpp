@@ -78,12 +78,16 @@ class Engine {
   QueueCtl *ctl_host = nullptr;  // pinned
   float4 *L = nullptr;
   uint64_t l_alloc = 0;
-  uint32_t q_alloc = 0;
+  uint32_t q_alloc = 0;   // nominal paths in flight (opt.queue when allocated)
+  uint32_t seg_cap = 0;   // slots per queue segment (kSegments segments)
+  uint64_t generation = 0;  // bumped whenever queue buffers are reallocated
+  uint32_t segment_slots() const;
+  static uint32_t queued(const QueueCtl &c, int parity);
   DevTicket *d_tickets = nullptr;
   uint32_t ticket_alloc = 0;
   unsigned long long *d_ticket_done = nullptr;
 
-  IterArgs iter_args(const DevJob *d_jobs, uint32_t n_tickets) const;
+  IterArgs iter_args(const DevJob *d_jobs) const;
   LaunchShape shape(int which) const;
   bool launch_iteration(const IterArgs &a, int parity, std::string *err);
   bool ensure_tickets(uint32_t n, std::string *err);
@@ -95,6 +99,9 @@ class Engine {
              const IterArgs &a, int parity);
   std::vector<hipEvent_t> ev_pool_;
 };
+
+// Message returned by rd_last_error(NULL) (calls with no renderer handle).
+void set_global_error(const std::string &msg);
 
 // Fills the camera / world / render fields of a DevJob from a scene.
 void fill_job(const rd_scene &s, const DeviceScene &ds, float *acc, uint32_t max_bounces,

@@ -3,6 +3,12 @@
 #include "kernels.hpp"
 #include "pathmath.hpp"
 
+// Target resident waves per SIMD for the latency-bound traversal kernel; the
+// compiler caps its registers to fit (without spilling, checked at build).
+#ifndef RDH_EXTEND_WAVES
+#define RDH_EXTEND_WAVES 8
+#endif
+
 namespace rdh {
 
 namespace {
@@ -13,26 +19,84 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
 }
 __device__ __forceinline__ uint32_t lane_id() { return lanes_below(~0ull); }
 
-// Work of one iteration: survivors [0, base) plus n_new fresh paths.
-struct IterPlan {
-  uint32_t base, n_new, n;
+// Work of one iteration over queue[p]: segment s holds cnt[s] survivors
+// followed by add[s] fresh paths (n_new in all, at most the nominal capacity
+// in total).  Every workgroup of generate, extend and shade derives the same
+// plan from the same counters, which no kernel of the iteration modifies.
+struct SegPlan {
+  uint32_t cnt[kSegments];
+  uint32_t add[kSegments];
+  uint32_t pre[kSegments + 1];      // prefix of cnt + add: consumers' linear order
+  uint32_t pre_new[kSegments + 1];  // prefix of add: generate's linear order
+  uint32_t n_new, n;
   unsigned long long cursor;
 };
-__device__ __forceinline__ IterPlan plan_of(const IterArgs &a, int p) {
-  IterPlan pl;
-  pl.base = a.ctl->count[p];
-  pl.cursor = a.ctl->cursor[p];
-  const unsigned long long avail = a.ctl->total - pl.cursor;
-  const unsigned long long room = (unsigned long long)(a.capacity - pl.base);
-  pl.n_new = (uint32_t)(avail < room ? avail : room);
-  pl.n = pl.base + pl.n_new;
-  return pl;
+
+__device__ __forceinline__ void make_plan(const IterArgs &a, int p, SegPlan &sp) {
+  if (threadIdx.x < kSegments) {
+    // a counter above seg_cap only follows an overflow (already flagged for
+    // the host); clamping keeps every slot index of this iteration in bounds
+    const uint32_t c = a.ctl->count[p][threadIdx.x * kSegStride];
+    sp.cnt[threadIdx.x] = c < a.seg_cap ? c : a.seg_cap;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t n_old = 0;
+    for (int s = 0; s < kSegments; ++s) n_old += sp.cnt[s];
+    const unsigned long long cursor = a.ctl->cursor[p];
+    const unsigned long long avail = a.ctl->total - cursor;
+    const unsigned long long room = (unsigned long long)(a.capacity - n_old);
+    const uint32_t n_new = (uint32_t)(avail < room ? avail : room);
+    uint32_t left = n_new;
+    sp.pre[0] = 0;
+    sp.pre_new[0] = 0;
+    for (int s = 0; s < kSegments; ++s) {
+      const uint32_t free_s = a.seg_cap - sp.cnt[s];
+      const uint32_t add = left < free_s ? left : free_s;
+      left -= add;
+      sp.add[s] = add;
+      sp.pre_new[s + 1] = sp.pre_new[s] + add;
+      sp.pre[s + 1] = sp.pre[s] + sp.cnt[s] + add;
+    }
+    sp.n_new = n_new;
+    sp.n = sp.pre[kSegments];
+    sp.cursor = cursor;
+  }
+  __syncthreads();
 }
 
-// Largest ticket index whose work_begin <= w (tickets sorted by work_begin).
-__device__ __forceinline__ uint32_t find_ticket(const DevTicket *t, uint32_t n,
+// Largest s with pre[s] <= i (pre non-decreasing, pre[0] = 0 <= i).
+__device__ __forceinline__ uint32_t segment_of(const uint32_t *pre, uint32_t i) {
+  uint32_t lo = 0, hi = kSegments;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (pre[mid] <= i) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// Queue slot of the i-th path of the iteration (consumers' order).  `s` is
+// the caller's running segment: a lane's index only grows, so its segment
+// only moves forward and each step costs one LDS read.
+__device__ __forceinline__ uint32_t slot_of(const SegPlan &sp, uint32_t seg_cap, uint32_t i,
+                                            uint32_t &s) {
+  while (sp.pre[s + 1] <= i) ++s;
+  return s * seg_cap + (i - sp.pre[s]);
+}
+
+// Queue of parity p.  Indexing a.q[p] with the runtime parity makes the
+// compiler fetch the pointer table through a vector address and keep all 28
+// pointers in VGPRs; a uniform select keeps them in scalar registers.
+__device__ __forceinline__ PathQueue queue_of(const IterArgs &a, int p) {
+  return p ? a.q[1] : a.q[0];
+}
+
+// Largest ticket index in [lo, n) whose work_begin <= w (tickets sorted by
+// work_begin; the host keeps lo at or below the ticket holding the cursor).
+__device__ __forceinline__ uint32_t find_ticket(const DevTicket *t, uint32_t lo, uint32_t n,
                                                 unsigned long long w) {
-  uint32_t lo = 0, hi = n;
+  uint32_t hi = n;
   while (hi - lo > 1) {
     const uint32_t mid = (lo + hi) >> 1;
     if (t[mid].work_begin <= w) lo = mid;
@@ -68,13 +132,13 @@ __device__ __forceinline__ void test_leaf(const RayPre &r, const DevPrim *prims,
 }
 
 // Closest hit over one scene.  STACK points at this thread's column of the
-// block's LDS stack (entry k at stack[k * STRIDE]).
-template <int STRIDE, bool COUNT>
+// block's LDS stack (entry k at stack[k * STRIDE]); StackT is uint16_t when
+// every BVH has at most 65535 nodes (half the LDS, more resident waves).
+template <int STRIDE, bool COUNT, class StackT>
 __device__ __forceinline__ void trace(const DevNode *__restrict__ nodes,
                                       const DevPrim *__restrict__ prims, uint32_t n_prims,
                                       uint32_t n_nodes, float r0, int brute, const RayPre &r,
-                                      int32_t *stack, float *bt, uint32_t *bid,
-                                      TraceCounts &tc) {
+                                      StackT *stack, float *bt, uint32_t *bid, TraceCounts &tc) {
   *bid = kNoHit;
   *bt = 0.0f;
   if (n_prims == 0) return;
@@ -116,7 +180,7 @@ __device__ __forceinline__ void trace(const DevNode *__restrict__ nodes,
     }
     if (h0 && h1) {
       const bool first0 = t0 <= t1;
-      stack[sp * STRIDE] = first0 ? ch.y : ch.x;
+      stack[sp * STRIDE] = (StackT)(first0 ? ch.y : ch.x);
       ++sp;
       node = first0 ? ch.x : ch.y;
     } else if (h0) {
@@ -126,29 +190,38 @@ __device__ __forceinline__ void trace(const DevNode *__restrict__ nodes,
     } else {
       if (sp == 0) break;
       --sp;
-      node = stack[sp * STRIDE];
+      node = (int32_t)stack[sp * STRIDE];
     }
   }
 }
 
+// Job and ticket of a path.  In single-job launches (the per-scene renderer)
+// both are the first table entries, so every lane reads the same address and
+// the loads become scalar.
+template <bool SINGLE>
+__device__ __forceinline__ uint32_t ticket_of(uint32_t tb) { return SINGLE ? 0u : (tb >> 8); }
+
 // ------------------------------------------------------------ generate ----
-template <bool COUNT>
+template <bool COUNT, bool SINGLE>
 __global__ void k_generate(IterArgs a, int p) {
-  const IterPlan pl = plan_of(a, p);
+  __shared__ SegPlan pl;
+  make_plan(a, p, pl);
   const uint32_t stride = gridDim.x * blockDim.x;
-  const PathQueue q = a.q[p];
+  const PathQueue q = queue_of(a, p);
+  const uint32_t t_lo = a.ctl->ticket_lo, t_n = a.ctl->n_tickets;
   uint32_t made = 0;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < pl.n_new; i += stride) {
     const unsigned long long w = pl.cursor + i;
-    const uint32_t ti = find_ticket(a.tickets, a.n_tickets, w);
+    const uint32_t s = segment_of(pl.pre_new, i);
+    const uint32_t k = s * a.seg_cap + pl.cnt[s] + (i - pl.pre_new[s]);
+    const uint32_t ti = SINGLE ? 0u : find_ticket(a.tickets, t_lo, t_n, w);
     const DevTicket &T = a.tickets[ti];
-    const DevJob &J = a.jobs[T.job];
+    const DevJob &J = a.jobs[SINGLE ? 0u : T.job];
     const uint32_t pidx = (uint32_t)(w - T.work_begin);
     const uint32_t npix = J.width * J.height;
     const uint32_t pixel = pidx % npix;
     F3 o, d;
     camera_ray(J, pixel % J.width, pixel / J.width, &o, &d);
-    const uint32_t k = pl.base + i;
     q.ox[k] = o.x; q.oy[k] = o.y; q.oz[k] = o.z;
     q.dx[k] = d.x; q.dy[k] = d.y; q.dz[k] = d.z;
     q.ax[k] = 1.0f; q.ay[k] = 1.0f; q.az[k] = 1.0f;
@@ -161,29 +234,31 @@ __global__ void k_generate(IterArgs a, int p) {
 }
 
 // -------------------------------------------------------------- extend ----
-template <int BLOCK, bool COUNT>
-__global__ __launch_bounds__(BLOCK) void k_extend(IterArgs a, int p) {
-  __shared__ int32_t stack[kStackDepth * BLOCK];
-  const IterPlan pl = plan_of(a, p);
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    a.ctl->count[1 - p] = 0;
-    a.ctl->cursor[1 - p] = pl.cursor + pl.n_new;
-  }
+template <int BLOCK, bool COUNT, bool SINGLE, class StackT>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(RDH_EXTEND_WAVES, 8)))
+void k_extend(IterArgs a, int p) {
+  __shared__ StackT stack[kStackDepth * BLOCK];
+  __shared__ SegPlan pl;
+  make_plan(a, p, pl);
+  // queue[1-p] is empty until shade fills it; nothing else reads it now
+  if (blockIdx.x == 0 && threadIdx.x < kSegments) a.ctl->count[1 - p][threadIdx.x * kSegStride] = 0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) a.ctl->cursor[1 - p] = pl.cursor + pl.n_new;
   const uint32_t stride = gridDim.x * BLOCK;
-  const PathQueue q = a.q[p];
+  const PathQueue q = queue_of(a, p);
   TraceCounts tc;
-  uint32_t rays = 0;
+  uint32_t rays = 0, seg = 0;
   for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < pl.n; i += stride) {
-    const F3 o = f3(q.ox[i], q.oy[i], q.oz[i]);
-    const F3 d = f3(q.dx[i], q.dy[i], q.dz[i]);
-    const DevJob &J = a.jobs[a.tickets[q.tb[i] >> 8].job];
+    const uint32_t k = slot_of(pl, a.seg_cap, i, seg);
+    const F3 o = f3(q.ox[k], q.oy[k], q.oz[k]);
+    const F3 d = f3(q.dx[k], q.dy[k], q.dz[k]);
+    const DevJob &J = a.jobs[SINGLE ? 0u : a.tickets[q.tb[k] >> 8].job];
     const RayPre r = ray_pre(o, d);
     float bt;
     uint32_t bid;
-    trace<BLOCK, COUNT>(J.nodes, J.prims, J.n_prims, J.n_nodes, J.r0_tol, a.brute_force, r,
-                        stack + threadIdx.x, &bt, &bid, tc);
-    a.hit_t[i] = bt;
-    a.hit_id[i] = bid;
+    trace<BLOCK, COUNT, StackT>(J.nodes, J.prims, J.n_prims, J.n_nodes, J.r0_tol, a.brute_force,
+                                r, stack + threadIdx.x, &bt, &bid, tc);
+    a.hit_t[k] = bt;
+    a.hit_id[k] = bid;
     ++rays;
   }
   if (COUNT) {
@@ -195,36 +270,48 @@ __global__ __launch_bounds__(BLOCK) void k_extend(IterArgs a, int p) {
 }
 
 // --------------------------------------------------------------- shade ----
-template <bool COUNT>
-__global__ void k_shade(IterArgs a, int p) {
-  const IterPlan pl = plan_of(a, p);
-  const PathQueue q = a.q[p], nq = a.q[1 - p];
+// Survivors are compacted per workgroup: a ballot + mbcnt gives each lane its
+// slot inside its wave, an LDS scan over the block's waves gives each wave its
+// offset, and one atomic per workgroup iteration reserves the block's range
+// in its own segment (blockIdx % kSegments) of the next queue.  The host
+// sizes segments so that the blocks sharing one can never overfill it; the
+// overflow flag is a guard, not a mode of operation.
+template <int BLOCK, bool COUNT, bool SINGLE>
+__global__ __launch_bounds__(BLOCK) void k_shade(IterArgs a, int p) {
+  constexpr int kWaves = BLOCK / 64;
+  __shared__ uint32_t wave_off[kWaves];
+  __shared__ uint32_t block_base;
+  __shared__ SegPlan pl;
+  make_plan(a, p, pl);
+  const PathQueue q = queue_of(a, p), nq = queue_of(a, 1 - p);
   const uint32_t lane = lane_id();
-  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const uint32_t n_waves = (gridDim.x * blockDim.x) >> 6;
-  uint32_t c_sph = 0, c_cube = 0, c_glass = 0, c_miss = 0;
-  for (uint32_t base = wave * 64u; base < pl.n; base += n_waves * 64u) {
-    const uint32_t i = base + lane;
+  const uint32_t wid = threadIdx.x >> 6;
+  const uint32_t out_seg = blockIdx.x % kSegments;
+  uint32_t *out_count = &a.ctl->count[1 - p][out_seg * kSegStride];
+  uint32_t c_sph = 0, c_cube = 0, c_glass = 0, c_miss = 0, seg = 0;
+  for (uint32_t base = blockIdx.x * BLOCK; base < pl.n; base += gridDim.x * BLOCK) {
+    const uint32_t i = base + threadIdx.x;
     bool alive = false, finished = false;
     uint32_t ti = 0, pidx = 0, bounce = 0;
     F3 no = f3(0, 0, 0), nd = no, na = no, nl = no;
     uint64_t l_slot = 0;
     if (i < pl.n) {
-      const F3 o = f3(q.ox[i], q.oy[i], q.oz[i]);
-      const F3 d = f3(q.dx[i], q.dy[i], q.dz[i]);
-      const F3 at = f3(q.ax[i], q.ay[i], q.az[i]);
-      const F3 li = f3(q.lx[i], q.ly[i], q.lz[i]);
-      const uint32_t tb = q.tb[i];
-      ti = tb >> 8;
+      const uint32_t k = slot_of(pl, a.seg_cap, i, seg);
+      const F3 o = f3(q.ox[k], q.oy[k], q.oz[k]);
+      const F3 d = f3(q.dx[k], q.dy[k], q.dz[k]);
+      const F3 at = f3(q.ax[k], q.ay[k], q.az[k]);
+      const F3 li = f3(q.lx[k], q.ly[k], q.lz[k]);
+      const uint32_t tb = q.tb[k];
+      ti = ticket_of<SINGLE>(tb);
       bounce = tb & 0xffu;
-      pidx = q.pidx[i];
+      pidx = q.pidx[k];
       const DevTicket &T = a.tickets[ti];
-      const DevJob &J = a.jobs[T.job];
+      const DevJob &J = a.jobs[SINGLE ? 0u : T.job];
       l_slot = T.l_offset + pidx;
-      const uint32_t id = a.hit_id[i];
+      const uint32_t id = a.hit_id[k];
       if (id != kNoHit) {
         const DevObject ob = J.objects[id];
-        const HitRec h = closest_hit(o, d, a.hit_t[i], ob);
+        const HitRec h = closest_hit(o, d, a.hit_t[k], ob);
         const uint32_t npix = J.width * J.height;
         BounceDraws dr;
         dr.init(J.seed, pidx % npix, T.sample_base + pidx / npix, bounce);
@@ -248,32 +335,47 @@ __global__ void k_shade(IterArgs a, int p) {
         if (COUNT) c_miss++;
       }
     }
-    // wave64 stream compaction of the survivors into q[1-p]
     const uint64_t am = __ballot(alive);
-    if (am) {
-      uint32_t wbase = 0;
-      if (lane == 0) wbase = atomicAdd(&a.ctl->count[1 - p], (uint32_t)__popcll(am));
-      wbase = __builtin_amdgcn_readfirstlane(wbase);
-      if (alive) {
-        const uint32_t k = wbase + lanes_below(am);
-        nq.ox[k] = no.x; nq.oy[k] = no.y; nq.oz[k] = no.z;
-        nq.dx[k] = nd.x; nq.dy[k] = nd.y; nq.dz[k] = nd.z;
-        nq.ax[k] = na.x; nq.ay[k] = na.y; nq.az[k] = na.z;
-        nq.lx[k] = nl.x; nq.ly[k] = nl.y; nq.lz[k] = nl.z;
-        nq.tb[k] = (ti << 8) | bounce;
-        nq.pidx[k] = pidx;
+    if (lane == 0) wave_off[wid] = (uint32_t)__popcll(am);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t tot = 0;
+#pragma unroll
+      for (int w = 0; w < kWaves; ++w) {
+        const uint32_t c = wave_off[w];
+        wave_off[w] = tot;
+        tot += c;
       }
+      uint32_t b0 = tot ? atomicAdd(out_count, tot) : 0u;
+      if (b0 + tot > a.seg_cap) {
+        a.ctl->overflow = 1u;  // never written past the segment: the host fails the render
+        b0 = 0xffffffffu;
+      }
+      block_base = b0;
+    }
+    __syncthreads();
+    if (alive && block_base != 0xffffffffu) {
+      const uint32_t k = out_seg * a.seg_cap + block_base + wave_off[wid] + lanes_below(am);
+      nq.ox[k] = no.x; nq.oy[k] = no.y; nq.oz[k] = no.z;
+      nq.dx[k] = nd.x; nq.dy[k] = nd.y; nq.dz[k] = nd.z;
+      nq.ax[k] = na.x; nq.ay[k] = na.y; nq.az[k] = na.z;
+      nq.lx[k] = nl.x; nq.ly[k] = nl.y; nq.lz[k] = nl.z;
+      nq.tb[k] = (ti << 8) | bounce;
+      nq.pidx[k] = pidx;
     }
     if (finished) a.L[l_slot] = make_float4(nl.x, nl.y, nl.z, 1.0f);
-    // finished-path counts per ticket, one atomic per distinct ticket per wave
-    uint64_t fm = __ballot(finished);
-    while (fm) {
-      const uint32_t leader = (uint32_t)(__ffsll((unsigned long long)fm) - 1);
-      const uint32_t lt = (uint32_t)__builtin_amdgcn_readlane((int)ti, (int)leader);
-      const uint64_t same = __ballot(finished && ti == lt);
-      if (lane == leader) atomicAdd(&a.ticket_done[lt], (unsigned long long)__popcll(same));
-      fm &= ~same;
+    if (a.track_done) {
+      // finished paths per ticket: one atomic per distinct ticket per wave
+      uint64_t fm = __ballot(finished);
+      while (fm) {
+        const uint32_t leader = (uint32_t)(__ffsll((unsigned long long)fm) - 1);
+        const uint32_t lt = (uint32_t)__builtin_amdgcn_readlane((int)ti, (int)leader);
+        const uint64_t same = __ballot(finished && ti == lt);
+        if (lane == leader) atomicAdd(&a.ticket_done[lt], (unsigned long long)__popcll(same));
+        fm &= ~same;
+      }
     }
+    __syncthreads();  // wave_off / block_base are rewritten next iteration
   }
   if (COUNT) {
     if (c_sph) atomicAdd(&a.counters->v[C_SPHERE_HIT], (unsigned long long)c_sph);
@@ -326,12 +428,26 @@ __global__ void k_tonemap(const DevJob *job, uint32_t n, float spp, uchar4 *out,
   if (COUNT && done) atomicAdd(&counters->v[C_TONEMAP], (unsigned long long)done);
 }
 
-__global__ void k_set_ctl(QueueCtl *ctl, unsigned long long total) {
-  ctl->count[0] = 0;
-  ctl->count[1] = 0;
+__global__ void k_set_ctl(QueueCtl *ctl, unsigned long long total, uint32_t n_tickets) {
+  for (int s = 0; s < kSegments; ++s) {
+    ctl->count[0][s * kSegStride] = 0;
+    ctl->count[1][s * kSegStride] = 0;
+  }
+  ctl->overflow = 0;
   ctl->cursor[0] = 0;
   ctl->cursor[1] = 0;
   ctl->total = total;
+  ctl->n_tickets = n_tickets;
+  ctl->ticket_lo = 0;
+}
+
+// Appends work between iterations (the streaming engine): never touches the
+// ping-pong fields the kernels own.
+__global__ void k_extend_work(QueueCtl *ctl, unsigned long long total, uint32_t n_tickets,
+                              uint32_t ticket_lo) {
+  ctl->total = total;
+  ctl->n_tickets = n_tickets;
+  ctl->ticket_lo = ticket_lo;
 }
 
 __global__ void k_zero(float *p, uint64_t n) {
@@ -354,8 +470,8 @@ __global__ void k_debug_trace(const DevNode *nodes, const DevPrim *prims, uint32
   float bt;
   uint32_t bid;
   TraceCounts tc;
-  trace<64, false>(nodes, prims, n_prims, n_nodes, r0, brute, r, stack + threadIdx.x, &bt, &bid,
-                   tc);
+  trace<64, false, int32_t>(nodes, prims, n_prims, n_nodes, r0, brute, r, stack + threadIdx.x,
+                            &bt, &bid, tc);
   if (bid == kNoHit) {
     ids[i] = -1;
     ts[i] = 0.0f;
@@ -422,57 +538,73 @@ uint32_t blocks_for(uint64_t n, uint32_t block) { return (uint32_t)((n + block -
 }  // namespace
 
 // ------------------------------------------------------------ launchers ---
-int occupancy_blocks(int which, int block) {
-  int n = 0;
-  hipError_t e = hipSuccess;
+// Kernel instance for a launch: block size, counting build, single-job
+// specialisation and stack width are template parameters.
+template <bool COUNT, bool SINGLE>
+static const void *extend_fn(int block, bool wide) {
+#define RDH_EXT(B)                                                               \
+  if (block == B)                                                                \
+    return wide ? (const void *)k_extend<B, COUNT, SINGLE, int32_t>              \
+                : (const void *)k_extend<B, COUNT, SINGLE, uint16_t>;
+  RDH_EXT(64)
+  RDH_EXT(128)
+  RDH_EXT(256)
+#undef RDH_EXT
+  return nullptr;
+}
+template <bool COUNT, bool SINGLE>
+static const void *shade_fn(int block) {
+  if (block == 64) return (const void *)k_shade<64, COUNT, SINGLE>;
+  if (block == 128) return (const void *)k_shade<128, COUNT, SINGLE>;
+  if (block == 256) return (const void *)k_shade<256, COUNT, SINGLE>;
+  return nullptr;
+}
+static const void *kernel_fn(int which, int block, bool count, bool single, bool wide) {
   if (which == 0) {
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_generate<false>, block, 0);
-  } else if (which == 2) {
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_shade<false>, block, 0);
-  } else if (block == 64) {
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_extend<64, false>, 64, 0);
-  } else if (block == 128) {
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_extend<128, false>, 128, 0);
-  } else {
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_extend<256, false>, 256, 0);
+    if (count) return single ? (const void *)k_generate<true, true> : (const void *)k_generate<true, false>;
+    return single ? (const void *)k_generate<false, true> : (const void *)k_generate<false, false>;
   }
-  if (e != hipSuccess) {
+  if (which == 1) {
+    if (count) return single ? extend_fn<true, true>(block, wide) : extend_fn<true, false>(block, wide);
+    return single ? extend_fn<false, true>(block, wide) : extend_fn<false, false>(block, wide);
+  }
+  if (count) return single ? shade_fn<true, true>(block) : shade_fn<true, false>(block);
+  return single ? shade_fn<false, true>(block) : shade_fn<false, false>(block);
+}
+
+int occupancy_blocks(int which, int block) {
+  const void *fn = kernel_fn(which, block, false, true, false);
+  int n = 0;
+  if (!fn || hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, block, 0) != hipSuccess) {
     (void)hipGetLastError();
     return 1;
   }
   return n;
 }
 
+static hipError_t launch_stage(int which, const IterArgs &a, int parity, const LaunchShape &s,
+                               bool count, hipStream_t st) {
+  const void *fn = kernel_fn(which, s.block, count, a.single_job != 0, a.wide_stack != 0);
+  if (!fn) return hipErrorInvalidValue;
+  IterArgs args = a;
+  int par = parity;
+  void *kargs[] = {&args, &par};
+  return hipLaunchKernel(fn, dim3(s.grid), dim3(s.block), kargs, 0, st);
+}
+
 hipError_t launch_generate(const IterArgs &a, int parity, const LaunchShape &s, bool count,
                            hipStream_t st) {
-  if (count) hipLaunchKernelGGL(k_generate<true>, dim3(s.grid), dim3(s.block), 0, st, a, parity);
-  else hipLaunchKernelGGL(k_generate<false>, dim3(s.grid), dim3(s.block), 0, st, a, parity);
-  return hipGetLastError();
+  return launch_stage(0, a, parity, s, count, st);
 }
 
 hipError_t launch_extend(const IterArgs &a, int parity, const LaunchShape &s, bool count,
                          hipStream_t st) {
-#define RDH_EXTEND(B)                                                                       \
-  case B:                                                                                   \
-    if (count) hipLaunchKernelGGL((k_extend<B, true>), dim3(s.grid), dim3(B), 0, st, a, parity); \
-    else hipLaunchKernelGGL((k_extend<B, false>), dim3(s.grid), dim3(B), 0, st, a, parity); \
-    break;
-  switch (s.block) {
-    RDH_EXTEND(64)
-    RDH_EXTEND(128)
-    RDH_EXTEND(256)
-    default:
-      return hipErrorInvalidValue;
-  }
-#undef RDH_EXTEND
-  return hipGetLastError();
+  return launch_stage(1, a, parity, s, count, st);
 }
 
 hipError_t launch_shade(const IterArgs &a, int parity, const LaunchShape &s, bool count,
                         hipStream_t st) {
-  if (count) hipLaunchKernelGGL(k_shade<true>, dim3(s.grid), dim3(s.block), 0, st, a, parity);
-  else hipLaunchKernelGGL(k_shade<false>, dim3(s.grid), dim3(s.block), 0, st, a, parity);
-  return hipGetLastError();
+  return launch_stage(2, a, parity, s, count, st);
 }
 
 hipError_t launch_accumulate(const DevJob *job, const DevTicket &t, const float4 *L,
@@ -489,8 +621,15 @@ hipError_t launch_tonemap(const DevJob *job, uint32_t n_pixels, float spp, uchar
   return hipGetLastError();
 }
 
-hipError_t launch_set_ctl(QueueCtl *ctl, unsigned long long total, hipStream_t st) {
-  hipLaunchKernelGGL(k_set_ctl, dim3(1), dim3(1), 0, st, ctl, total);
+hipError_t launch_set_ctl(QueueCtl *ctl, unsigned long long total, uint32_t n_tickets,
+                          hipStream_t st) {
+  hipLaunchKernelGGL(k_set_ctl, dim3(1), dim3(1), 0, st, ctl, total, n_tickets);
+  return hipGetLastError();
+}
+
+hipError_t launch_extend_work(QueueCtl *ctl, unsigned long long total, uint32_t n_tickets,
+                              uint32_t ticket_lo, hipStream_t st) {
+  hipLaunchKernelGGL(k_extend_work, dim3(1), dim3(1), 0, st, ctl, total, n_tickets, ticket_lo);
   return hipGetLastError();
 }
 

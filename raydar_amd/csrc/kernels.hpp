@@ -27,13 +27,16 @@ struct IterArgs {
   uint32_t *hit_id;
   QueueCtl *ctl;
   const DevJob *jobs;
-  const DevTicket *tickets;
-  uint32_t n_tickets;
-  uint32_t capacity;
+  const DevTicket *tickets;         // the ticket count lives in QueueCtl
+  uint32_t capacity;                // paths in flight, all segments together
+  uint32_t seg_cap;                 // slots per queue segment
   float4 *L;                        // radiance arena (rgb, 1)
   unsigned long long *ticket_done;  // finished paths per ticket
   DevCounters *counters;            // used by the counting build only
   int brute_force;                  // 1: linear scan instead of the BVH
+  int single_job;                   // 1: every path belongs to job 0 / ticket 0
+  int track_done;                   // 1: count finished paths per ticket
+  int wide_stack;                   // 1: some BVH has more than 65535 nodes
 };
 
 struct LaunchShape {
@@ -54,7 +57,10 @@ hipError_t launch_tonemap(const DevJob *job, uint32_t n_pixels, float spp, uchar
 // Workgroups of kernel `which` (0 generate, 1 extend, 2 shade) that fit on one CU.
 int occupancy_blocks(int which, int block);
 
-hipError_t launch_set_ctl(QueueCtl *ctl, unsigned long long total, hipStream_t st);
+hipError_t launch_set_ctl(QueueCtl *ctl, unsigned long long total, uint32_t n_tickets,
+                          hipStream_t st);
+hipError_t launch_extend_work(QueueCtl *ctl, unsigned long long total, uint32_t n_tickets,
+                              uint32_t ticket_lo, hipStream_t st);
 hipError_t launch_zero_acc(float *acc, uint64_t n_floats, hipStream_t st);
 
 // Parity-test entry kernels (single rays / single hits).

@@ -26,7 +26,8 @@
 namespace rdh {
 
 constexpr int kMaxLeafSize = 8;
-constexpr int kStackDepth = 40;          // >= max BVH depth the builder emits
+constexpr int kStackDepth = 32;          // >= max BVH depth the builder emits
+constexpr uint32_t kNarrowStackNodes = 65535;  // 16-bit LDS stack entries suffice
 constexpr uint32_t kNoHit = 0x7fffffffu; // best-id sentinel: "no hit yet"
 
 // Leaf child code: bit 31 set, first prim in bits 4..30, count in bits 0..3.
@@ -104,12 +105,25 @@ struct PathQueue {
 };
 
 // Per-iteration control block in device memory (ping-pong by parity).
+// Each path queue is split into kSegments segments of seg_cap slots.  Shade
+// workgroup b appends its survivors to segment b % kSegments, so the
+// reservation atomics spread over kSegments counters on separate cache lines
+// instead of serialising on one address.  Consumers see the queue as the
+// concatenation of the segments' filled prefixes.
+constexpr int kSegments = 32;
+constexpr int kSegStride = 32;  // uint32 stride between counters: 128 bytes
+
+// The host only changes n_tickets, ticket_lo and total, between iterations.
 struct QueueCtl {
-  uint32_t count[2];        // paths in queue[p] at the start of an iteration
-  uint32_t pad0[2];
+  uint32_t n_tickets;       // tickets visible to generate
+  uint32_t ticket_lo;       // no path before this ticket is still to be generated
+  uint32_t overflow;        // set if a segment bound was ever exceeded (internal error)
+  uint32_t pad0;
   unsigned long long cursor[2];  // next global work index to generate
   unsigned long long total;      // end of the work list
   unsigned long long pad1;
+  uint32_t pad2[24];             // counters start on their own cache line
+  uint32_t count[2][kSegments * kSegStride];  // paths per segment of queue[p]
 };
 
 // Algorithmic event counters (collected only by the counting build).

@@ -77,25 +77,25 @@ RD_HD U4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint3
 // offset, 6 transmission test, 7 Fresnel test (glass) or metallic test,
 // 8-10 refraction offset (glass), 8 roughness test (non-metal, non-glass).
 // Blocks 0 and 1 are always consumed; block 2 is generated on demand.
+// Each block is generated the first time one of its slots is read: a draw's
+// value depends only on its counter, so skipping unread blocks changes no
+// result.
 struct BounceDraws {
   uint32_t pixel, sample, bounce, k0, k1;
-  U4 b0, b1, b2;
-  bool have2;
+  U4 b[3];
+  uint32_t have;
   RD_HD void init(uint64_t seed, uint32_t px, uint32_t smp, uint32_t bnc) {
     pixel = px; sample = smp; bounce = bnc;
     k0 = (uint32_t)seed; k1 = (uint32_t)(seed >> 32);
-    b0 = philox4x32_10(pixel, sample, bounce * 4u + 0u, 0u, k0, k1);
-    b1 = philox4x32_10(pixel, sample, bounce * 4u + 1u, 0u, k0, k1);
-    have2 = false;
+    have = 0;
   }
   RD_HD uint32_t slot(int s) {
-    if (s < 4) return b0.v[s];
-    if (s < 8) return b1.v[s - 4];
-    if (!have2) {
-      b2 = philox4x32_10(pixel, sample, bounce * 4u + 2u, 0u, k0, k1);
-      have2 = true;
+    const int blk = s >> 2;
+    if (!(have & (1u << blk))) {
+      b[blk] = philox4x32_10(pixel, sample, bounce * 4u + (uint32_t)blk, 0u, k0, k1);
+      have |= 1u << blk;
     }
-    return b2.v[s - 8];
+    return b[blk].v[s & 3];
   }
 };
 
@@ -300,24 +300,25 @@ struct ScatterOut {
   bool transmission;
 };
 
-// cpu.rs:262-333 for one hit.
+// cpu.rs:262-333 for one hit.  The reference computes the diffuse and the
+// specular direction before choosing; here the choice (which depends only on
+// draws 6-8, the material and can_refract) comes first and only the chosen
+// direction is computed, with the same operations.  Draws are keyed by slot,
+// so the result is identical.
 template <class Draws>
 RD_HD ScatterOut scatter(F3 ray_d, const HitRec &h, const DevObject &ob, Draws &dr, F3 atten,
                          F3 light) {
   const float rough = ob.m0[3];  // roughness^2, precomputed with the same multiply
   const float metallic = ob.m1[3];
   const float transmission = ob.m2[0];
-  F3 diffuse = add(h.n, unit_sphere_draw(dr, 0));
-  if (dot(diffuse, h.n) < 0.0f) diffuse = neg(diffuse);
-  const F3 perfect = reflect(ray_d, h.n);
-  const F3 offset = scale(unit_sphere_draw(dr, 3), rough);
-  const F3 specular = normalize(add(perfect, offset));
   const bool trans = unit_f32(dr.slot(6)) < transmission;
-  F3 dir;
+  enum { kSpecular, kDiffuse, kRefract } pick = kSpecular;
+  F3 rd = f3(0.0f, 0.0f, 0.0f);
+  float eta = 1.0f;
   if (trans) {
     float ior = ob.m2[1];
     if (h.front) ior = 1.0f / ior;
-    const F3 rd = normalize(ray_d);
+    rd = normalize(ray_d);
     const float cos_t = fminf(dot(rd, neg(h.n)), 1.0f);
     const float q = (ior - 1.0f) / (ior + 1.0f);
     const float r0 = q * q;
@@ -325,18 +326,23 @@ RD_HD ScatterOut scatter(F3 ray_d, const HitRec &h, const DevObject &ob, Draws &
     const float y2 = y * y;
     const float y5 = y * (y2 * y2);
     const float refl = r0 + (1.0f - r0) * y5;
-    if (refl < unit_f32(dr.slot(7)) && can_refract(rd, h.n, ior)) {
-      const F3 refr = refract(rd, h.n, ior);
-      dir = normalize(add(refr, scale(unit_sphere_draw(dr, 8), rough)));
-    } else {
-      dir = specular;
-    }
+    if (refl < unit_f32(dr.slot(7)) && can_refract(rd, h.n, ior)) pick = kRefract;
+    eta = ior;
   } else if (unit_f32(dr.slot(7)) < metallic) {
-    dir = specular;
+    pick = kSpecular;
   } else if (unit_f32(dr.slot(8)) < rough) {
-    dir = diffuse;
+    pick = kDiffuse;
+  }
+  F3 dir;
+  if (pick == kRefract) {
+    const F3 refr = refract(rd, h.n, eta);
+    dir = normalize(add(refr, scale(unit_sphere_draw(dr, 8), rough)));
+  } else if (pick == kDiffuse) {
+    dir = add(h.n, unit_sphere_draw(dr, 0));
+    if (dot(dir, h.n) < 0.0f) dir = neg(dir);
   } else {
-    dir = specular;
+    const F3 perfect = reflect(ray_d, h.n);
+    dir = normalize(add(perfect, scale(unit_sphere_draw(dr, 3), rough)));
   }
   ScatterOut out;
   const F3 off = trans ? dir : h.n;

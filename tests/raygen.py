@@ -69,3 +69,40 @@ ALL = (camera, secondary, grazing, axis_parallel, far)
 def mixed(scene, n_each, seed=0):
     rng = np.random.default_rng(seed)
     return np.concatenate([g(scene, n_each, rng) for g in ALL], 0)
+
+
+def material_zoo(seed=3, n=40):
+    """Random material parameters over every branch of cpu.rs:280-314."""
+    from raydar_amd import abi
+    from raydar_amd.scene import Scene, make_camera, make_object, sky_world
+    rng = np.random.default_rng(seed)
+    objs = []
+    for i in range(n):
+        kind = abi.RD_CUBE if i % 3 == 2 else abi.RD_SPHERE
+        size = float(rng.uniform(0.2, 1.5))
+        c = (float(rng.uniform(-6, 6)), float(rng.uniform(-1, 3)), float(rng.uniform(-6, 6)))
+        objs.append(make_object(
+            kind, c, size, albedo=rng.uniform(0, 1, 3), roughness=float(rng.choice([0, 1, rng.uniform()])),
+            metallic=float(rng.choice([0, 1, rng.uniform()])),
+            transmission=float(rng.choice([0, 1, rng.uniform()])), ior=float(rng.uniform(1.0, 2.5)),
+            emission_color=rng.uniform(0, 1, 3), emission_strength=float(rng.choice([0, 0, 5.0]))))
+    cam = make_camera((0, 4, 14), (0, 0, 0), (0, 1, 0), 160, 120, fov_deg=50)
+    return Scene(cam, sky_world(), np.array(objs))
+
+
+def scatter_inputs(scene, n, seed):
+    """Hit records (oracle closest hits of camera and secondary rays) with
+    random attenuation, light and draw words, in or_scatter_f32's layout."""
+    rng = np.random.default_rng(seed)
+    rays = np.concatenate([camera(scene, n, rng), secondary(scene, n, rng)])
+    ids, ts, _, _ = oracle.trace(scene, rays)
+    h = ids >= 0
+    m = int(h.sum())
+    packed = np.zeros((m, 16), np.float32)
+    packed[:, :6] = rays[h]
+    packed[:, 6] = ts[h]
+    packed[:, 7] = ids[h].astype(np.uint32).view(np.float32)
+    packed[:, 8:11] = rng.uniform(0, 1, (m, 3))
+    packed[:, 11:14] = rng.uniform(0, 3, (m, 3))
+    bits = rng.integers(0, 2**32, (m, 12), dtype=np.uint64).astype(np.uint32)
+    return packed, bits

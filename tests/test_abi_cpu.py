@@ -161,6 +161,32 @@ def test_bvh_traversal_matches_linear_scan(scene_name, leaf):
     np.testing.assert_array_equal(ts_h[hit].view(np.uint32), ts_o[hit].view(np.uint32))
 
 
+@pytest.mark.parametrize("scene_name", ["plain_small", "plain_1080p", "default", "zoo", "benchmark"])
+def test_kernel_scatter_host_build_matches_oracle(scene_name):
+    """pathmath.hpp's scatter (branch chosen first, only the chosen direction
+    computed, draws generated lazily) equals the oracle's eager restatement
+    of cpu.rs:262-333 bit for bit."""
+    sc = {"plain_small": lambda: scenegen.plain_small(0),
+          "plain_1080p": lambda: scenegen.plain_1080p(1),
+          "default": lambda: Scene.load(os.path.join(GOLDEN, "default.rscn")),
+          "zoo": raygen.material_zoo,
+          "benchmark": lambda: Scene.load(os.path.join(GOLDEN, "benchmark.rscn"))}[scene_name]()
+    packed, bits = raygen.scatter_inputs(sc, 4000, 21)
+    h = ra.host_scatter(sc, packed, bits)
+    o = oracle.scatter(sc, packed, bits)
+    np.testing.assert_array_equal(h.view(np.uint32), o.view(np.uint32))
+
+
+@pytest.mark.parametrize("scene_name", ["plain_small", "default", "benchmark"])
+def test_kernel_camera_host_build_matches_oracle(scene_name):
+    sc = {"plain_small": lambda: scenegen.plain_small(0),
+          "default": lambda: Scene.load(os.path.join(GOLDEN, "default.rscn")),
+          "benchmark": lambda: Scene.load(os.path.join(GOLDEN, "benchmark.rscn"))}[scene_name]()
+    pix = np.random.default_rng(2).integers(0, sc.width * sc.height, 5000).astype(np.uint32)
+    np.testing.assert_array_equal(ra.host_camera_rays(sc, pix).view(np.uint32),
+                                  oracle.camera_rays(sc, pix).view(np.uint32))
+
+
 def test_nonfinite_rays_use_the_scan():
     sc = scenegen.plain_1080p(1)
     rays = raygen.mixed(sc, 50, seed=9)

@@ -1,0 +1,141 @@
+"""Batched renderer and render server on the GPU.
+
+Every job's image depends only on (scene, seed, samples, bounces): the tests
+render jobs alone, mixed with others, split into passes, through the
+captured graph and through direct launches, and require bit-identical
+accumulators, equal to the oracle's FP32 render of the same job."""
+import os
+import time
+
+import numpy as np
+import pytest
+
+import oracle
+import raydar_amd as ra
+from raydar_amd import abi, scenegen
+from raydar_amd.batch import BatchRenderer, Job, RenderServer
+from raydar_amd.scene import Scene, make_camera, sky_world
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _jobs():
+    empty = Scene(make_camera((0, 1, 5), (0, 0, 0), (0, 1, 0), 40, 30, fov_deg=60), sky_world(),
+                  np.zeros(0, abi.OBJECT_DTYPE))
+    return [
+        (scenegen.plain_small(0), 4, 6, 11),
+        (scenegen.clustered(5, 300, 96, 64), 3, 12, 12),
+        (scenegen.plain_1080p(1, 120, 68), 2, 12, 13),
+        (Scene.load(os.path.join(GOLDEN, "default.rscn")), 1, 12, 14),
+        (empty, 2, 4, 15),
+    ]
+
+
+def _oracle_acc(scene, spp, bounces, seed):
+    return oracle.render(scene, spp, bounces, seed)
+
+
+def test_batch_matches_oracle_and_plain():
+    specs = _jobs()
+    jobs = [Job(s, spp, b, seed) for s, spp, b, seed in specs]
+    with BatchRenderer() as br:
+        st = br.render(jobs)
+    assert st["paths"] == sum(j.paths for j in jobs)
+    for job, (s, spp, b, seed) in zip(jobs, specs):
+        ref = _oracle_acc(s, spp, b, seed)
+        np.testing.assert_array_equal(job.acc.view(np.uint32), ref.view(np.uint32))
+        np.testing.assert_array_equal(job.rgba.reshape(-1), oracle.tonemap(ref, spp))
+        with ra.HipRenderer(max_sample_count=spp, max_bounces=b, seed=seed) as r:
+            r.render_frame(s)
+            np.testing.assert_array_equal(r.accumulator(), job.acc)
+
+
+@pytest.mark.parametrize("opts", [dict(graph=0), dict(graph=1, block=128, grid=97),
+                                  dict(block=64, leaf=1), dict(queue=4096)])
+def test_batch_launch_shapes_and_passes_agree(opts):
+    specs = _jobs()[:3]
+    a = [Job(s, spp, b, seed) for s, spp, b, seed in specs]
+    b_ = [Job(s, spp, b, seed, pass_samples=1) for s, spp, b, seed in specs]
+    with BatchRenderer(**opts) as br:
+        br.render(a)
+    with BatchRenderer() as br:
+        br.render(b_)
+    for x, y in zip(a, b_):
+        np.testing.assert_array_equal(x.acc, y.acc)
+        np.testing.assert_array_equal(x.rgba, y.rgba)
+
+
+def test_server_alone_equals_mixed():
+    specs = _jobs()
+    target = specs[1]
+    with RenderServer(pass_samples=1) as srv:
+        alone = Job(*target[:1], target[1], target[2], target[3])
+        jid = srv.submit(alone)
+        r = srv.wait(jid, 60000)
+        assert r["done"] == 1 and r["status"] == 0 and r["latency_ns"] > 0
+    with RenderServer(pass_samples=2) as srv:
+        mixed = [Job(s, spp, b, seed, progress=True) for s, spp, b, seed in specs]
+        ids = []
+        for j in mixed:
+            ids.append(srv.submit(j))
+            time.sleep(0.002)  # arrivals while others are on the GPU
+        res = [srv.wait(i, 60000) for i in ids]
+        st = srv.stats()
+    assert all(r["done"] == 1 and r["status"] == 0 for r in res)
+    np.testing.assert_array_equal(mixed[1].acc, alone.acc)
+    np.testing.assert_array_equal(mixed[1].rgba, alone.rgba)
+    for j in mixed:
+        np.testing.assert_array_equal(j.progress, j.rgba)  # last progressive image = final
+    assert st["jobs_done"] == len(specs) and st["paths"] == sum(j.paths for j in mixed)
+    ref = _oracle_acc(*target)
+    np.testing.assert_array_equal(alone.acc.view(np.uint32), ref.view(np.uint32))
+
+
+def test_server_budget_admission():
+    sc = scenegen.plain_small(0)
+    with pytest.raises(ra.RaydarError) as e:
+        RenderServer(hbm_budget_bytes=1 << 20)
+    assert e.value.status == abi.RD_ERR_BUDGET
+    with RenderServer(queue=65536) as probe:
+        fixed = probe.stats()["engine_bytes"]
+        jid = probe.submit(Job(sc, 1, 1, 0))
+        job_bytes = probe.wait(jid, 60000)["device_bytes"]
+    assert fixed > 0 and job_bytes > 256 * 256 * 20
+    # room for the engine and one job: the others wait for admission
+    with RenderServer(hbm_budget_bytes=fixed + job_bytes * 3 // 2, queue=65536) as srv:
+        jobs = [Job(sc, 2, 4, s) for s in range(6)]
+        ids = [srv.submit(j) for j in jobs]
+        res = [srv.wait(i, 60000) for i in ids]
+        st = srv.stats()
+    assert all(r["done"] and r["status"] == 0 for r in res)
+    assert st["max_resident"] == 1 and st["hbm_peak"] <= job_bytes
+    assert max(r["queued_ns"] for r in res) > 0
+    for j, s in zip(jobs, range(6)):
+        np.testing.assert_array_equal(j.acc, _oracle_acc(sc, 2, 4, s))
+
+
+def test_server_rejects_invalid_jobs():
+    sc = scenegen.plain_small(0)
+    with RenderServer(queue=65536) as srv:
+        with pytest.raises(ra.RaydarError) as e:
+            srv.submit(Job(sc, 0, 4, 0))
+        assert e.value.status == abi.RD_ERR_INVALID_ARGUMENT
+        t = scenegen.plain_small(0)
+        t.world = dict(t.world, kind=abi.RD_WORLD_TRANSPARENT)
+        with pytest.raises(ra.RaydarError) as e:
+            srv.submit(Job(t, 1, 4, 0))
+        assert e.value.status == abi.RD_ERR_UNSUPPORTED
+        assert srv.stats()["jobs_rejected"] == 2
+
+
+def test_autotune_returns_a_valid_setting():
+    jobs = [Job(scenegen.plain_small(0), 1, 4, 0), Job(scenegen.clustered(5, 200, 64, 64), 1, 4, 1)]
+    with BatchRenderer() as br:
+        r = br.autotune(jobs)
+        assert r["block"] in (64, 128, 256) and r["leaf"] in (1, 2, 4, 8)
+        assert r["configs_tried"] == 48 and r["best_ms"] <= r["default_ms"]
+        out = [Job(scenegen.plain_small(0), 2, 4, 0)]
+        br.render(out)
+    np.testing.assert_array_equal(out[0].acc, _oracle_acc(scenegen.plain_small(0), 2, 4, 0))

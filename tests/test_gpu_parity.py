@@ -16,8 +16,8 @@ import pytest
 
 import oracle
 import raydar_amd as ra
-from raydar_amd import abi, scenegen
-from raydar_amd.scene import Scene, make_camera, make_object, sky_world
+from raydar_amd import scenegen
+from raydar_amd.scene import Scene
 from tests import raygen
 
 pytestmark = pytest.mark.gpu
@@ -32,22 +32,7 @@ def _close(a, b, rel=REL):
     return np.abs(a - b) <= rel * scale
 
 
-def material_zoo(seed=3, n=40):
-    """Random material parameters over every branch of cpu.rs:280-314."""
-    rng = np.random.default_rng(seed)
-    objs = []
-    for i in range(n):
-        kind = abi.RD_CUBE if i % 3 == 2 else abi.RD_SPHERE
-        size = float(rng.uniform(0.2, 1.5))
-        c = (float(rng.uniform(-6, 6)), float(rng.uniform(-1, 3)), float(rng.uniform(-6, 6)))
-        objs.append(make_object(
-            kind, c, size, albedo=rng.uniform(0, 1, 3), roughness=float(rng.choice([0, 1, rng.uniform()])),
-            metallic=float(rng.choice([0, 1, rng.uniform()])),
-            transmission=float(rng.choice([0, 1, rng.uniform()])), ior=float(rng.uniform(1.0, 2.5)),
-            emission_color=rng.uniform(0, 1, 3), emission_strength=float(rng.choice([0, 0, 5.0]))))
-    cam = make_camera((0, 4, 14), (0, 0, 0), (0, 1, 0), 160, 120, fov_deg=50)
-    return Scene(cam, sky_world(), np.array(objs))
-
+material_zoo = raygen.material_zoo
 
 SCENES = {
     "plain_small": lambda: scenegen.plain_small(0),
@@ -109,20 +94,7 @@ def test_closest_hit_parity(name, leaf, brute):
     assert exact == 1.0
 
 
-def _scatter_inputs(sc, n, seed):
-    rng = np.random.default_rng(seed)
-    rays = np.concatenate([raygen.camera(sc, n, rng), raygen.secondary(sc, n, rng)])
-    ids, ts, _, _ = oracle.trace(sc, rays)
-    h = ids >= 0
-    m = int(h.sum())
-    packed = np.zeros((m, 16), np.float32)
-    packed[:, :6] = rays[h]
-    packed[:, 6] = ts[h]
-    packed[:, 7] = ids[h].astype(np.uint32).view(np.float32)
-    packed[:, 8:11] = rng.uniform(0, 1, (m, 3))
-    packed[:, 11:14] = rng.uniform(0, 3, (m, 3))
-    bits = rng.integers(0, 2**32, (m, 12), dtype=np.uint64).astype(np.uint32)
-    return packed, bits
+_scatter_inputs = raygen.scatter_inputs
 
 
 @pytest.mark.parametrize("name", ["plain_small", "plain_1080p", "default_rscn", "material_zoo",
