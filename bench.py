@@ -88,6 +88,56 @@ def cpu_baseline(job, budget_s=10.0):
     }
 
 
+def batched_vs_plain(autotune=True):
+    """64 clustered scenes (16-2048 primitives, 128^2..1080p, 8/32/128 spp,
+    4/8/12 bounces) in one batched wavefront vs 64 sequential plain renders
+    on one reused per-scene renderer (new_frame + render_frame each)."""
+    import raydar_amd as ra
+    from raydar_amd import scenegen
+    from raydar_amd.batch import BatchRenderer, Job
+    specs = scenegen.batch_jobs()
+    paths = sum(j.paths for j in specs)
+    out = {"jobs": len(specs), "paths": paths}
+    with BatchRenderer() as br:
+        if autotune:
+            sample = [Job(j.scene, 1, j.bounces, j.seed, accumulator=False) for j in specs[::8]]
+            t0 = time.perf_counter()
+            tune = br.autotune(sample)
+            out["autotune"] = dict(tune, seconds=round(time.perf_counter() - t0, 2))
+        br.render([Job(specs[0].scene, 1, 4, 0, accumulator=False)])  # warm the launch shapes
+        jobs = [Job(j.scene, j.spp, j.bounces, j.seed, accumulator=False) for j in specs]
+        t0 = time.perf_counter()
+        st = br.render(jobs)
+        dt_b = time.perf_counter() - t0
+        out["batched"] = {"seconds": round(dt_b, 4), "mpaths_per_s": round(paths / dt_b / 1e6, 2),
+                          "iterations": st["iterations"], "graph_launches": st["graph_launches"],
+                          "options": {k: br.get_option(k) for k in ("block", "queue", "leaf")}}
+    with ra.HipRenderer() as r:
+        r.max_sample_count = 1
+        r.render_frame(specs[0].scene)  # warm-up
+        t0 = time.perf_counter()
+        for j in specs:
+            r.max_sample_count = j.spp
+            r.max_bounces = j.bounces
+            r.set_seed(j.seed)
+            r.render_frame(j.scene)
+        dt_p = time.perf_counter() - t0
+    out["sequential_plain"] = {"seconds": round(dt_p, 4), "mpaths_per_s": round(paths / dt_p / 1e6, 2)}
+    out["batched_over_plain"] = round(dt_p / dt_b, 3)
+    return out
+
+
+def serving(seconds=60.0, rate=20.0):
+    """Poisson request stream against the render server (200 GB HBM budget,
+    8-spp progressive passes, scenes as the batch with 256^2..1080p,
+    32/64/128 spp, 12 bounces); latency = submit to final image."""
+    from raydar_amd import scenegen, serving as srv
+    arrivals = scenegen.serve_arrivals(rate=rate, duration=seconds)
+    rep = srv.run(arrivals, hbm_budget_bytes=200 << 30, pass_samples=8)
+    rep["config"] = f"Poisson {rate}/s for {seconds} s, arrival seed 7, scene seeds 2000+i"
+    return rep
+
+
 def _cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -107,6 +157,11 @@ def main():
     ap.add_argument("--workload", default="plain_1080p")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=10.0)
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--serve-seconds", type=float, default=30.0,
+                    help="length of the Poisson serving run (0 skips it)")
+    ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--option", action="append", default=[], help="key=value renderer option")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -209,6 +264,12 @@ def main():
         "counters_per_step": cnt,
     }
     r.close()
+    if not args.no_batched:
+        print("# batched vs plain...", file=sys.stderr, flush=True)
+        result["batched"] = batched_vs_plain(autotune=not args.no_autotune)
+    if args.serve_seconds > 0:
+        print(f"# serving {args.serve_seconds} s...", file=sys.stderr, flush=True)
+        result["serving"] = serving(args.serve_seconds, args.serve_rate)
     if not args.no_cpu_baseline:
         print("# cpu baseline...", file=sys.stderr, flush=True)
         cb = cpu_baseline(job, args.cpu_budget)

@@ -193,7 +193,7 @@ rd_status rd_read_accumulator(rd_renderer *r, float *out, size_t float_capacity)
 /* Tuning knobs, normally chosen by the autotuner.  Keys:
  *   "block"       workgroup size of the wavefront kernels: 64, 128 or 256
  *   "queue"       path-queue capacity (paths in flight), >= 1024
- *   "leaf"        BVH leaf size 1..8 (takes effect at the next new_frame)
+ *   "leaf"        BVH leaf size 1..8, default 1 (takes effect at the next new_frame)
  *   "grid"        workgroups per persistent launch (0 = automatic)
  *   "traversal"   0 = BVH, 1 = brute-force linear scan (cpu.rs:345-349)
  *   "count"       1 = collect algorithmic event counters in the next passes */

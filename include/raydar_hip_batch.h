@@ -46,11 +46,13 @@ typedef struct rd_batch_stats {
   uint64_t iterations;    /* generate/extend/shade rounds                      */
   uint64_t graph_launches;
   uint64_t pumps;         /* host synchronisations                             */
+  int64_t build_ns;       /* host BVH build time, summed over the jobs         */
+  uint32_t host_threads;  /* threads that built the jobs' scenes               */
 } rd_batch_stats;
 
 rd_status rd_batch_create(rd_batch **out);
 void rd_batch_destroy(rd_batch *b);
-/* Keys: "block" (64/128/256), "queue" (paths in flight), "leaf" (1..8),
+/* Keys: "block" (64/128/256), "queue" (paths in flight), "leaf" (1..8, default 2),
  * "grid" (workgroups, 0 = auto), "graph" (1 = replay a captured hipGraph),
  * "l_capacity" (radiance slots), "iters_per_pump" (even, >= 2). */
 rd_status rd_batch_set_option(rd_batch *b, const char *key, int64_t value);

@@ -15,7 +15,8 @@ class RdJobDesc(C.Structure):
 
 class RdBatchStats(C.Structure):
     _fields_ = [("elapsed_ns", C.c_int64), ("paths", C.c_uint64), ("iterations", C.c_uint64),
-                ("graph_launches", C.c_uint64), ("pumps", C.c_uint64)]
+                ("graph_launches", C.c_uint64), ("pumps", C.c_uint64), ("build_ns", C.c_int64),
+                ("host_threads", C.c_uint32)]
 
 
 class RdTuneResult(C.Structure):
@@ -81,14 +82,16 @@ def _bind():
 class Job:
     """A render job plus its output buffers (kept alive with the job)."""
 
-    def __init__(self, scene, samples, max_bounces=12, seed=0, pass_samples=0, progress=False):
+    def __init__(self, scene, samples, max_bounces=12, seed=0, pass_samples=0, progress=False,
+                 accumulator=True):
         self.scene = scene
         self.rd_scene = scene.as_rd()
         self.rgba = np.zeros((scene.height, scene.width, 4), np.uint8)
-        self.acc = np.zeros((scene.height * scene.width, 4), np.float32)
+        self.acc = np.zeros((scene.height * scene.width, 4), np.float32) if accumulator else None
         self.progress = np.zeros_like(self.rgba) if progress else None
         self.desc = RdJobDesc(C.pointer(self.rd_scene), samples, max_bounces, seed, pass_samples,
-                              self.rgba.ctypes.data, self.acc.ctypes.data,
+                              self.rgba.ctypes.data,
+                              self.acc.ctypes.data if accumulator else None,
                               self.progress.ctypes.data if progress else None)
 
     @property

@@ -14,7 +14,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
+#include <cmath>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
@@ -287,8 +289,9 @@ class Streamer {
   }
 
   bool issue() {
-    // Keep about two queue loads of work ahead of the cursor.
-    const uint64_t ahead = 2ull * eng.q_alloc;
+    // Keep enough work ahead of the cursor that no iteration of the next
+    // pump finds the work list empty (each can admit a queue load).
+    const uint64_t ahead = (uint64_t)(iters_per_pump + 1) * eng.q_alloc;
     bool added = false;
     for (SJob *j : resident_) {
       while (j->passes_issued < j->passes_total && ctl_total_ - last_cursor_ < ahead) {
@@ -555,24 +558,97 @@ bool engine_option_get(const rdh::Streamer &st, const std::string &k, int64_t *v
   return true;
 }
 
+// Cheap checks of every job before any work starts, so a bad job fails the
+// call before anything is rendered.
+bool validate_job(const rd_job_desc &d, std::string *e, rd_status *st) {
+  *st = RD_ERR_INVALID_ARGUMENT;
+  if (!d.scene) { *e = "job scene is NULL"; return false; }
+  const rd_scene &s = *d.scene;
+  if (s.camera.resolution_x == 0 || s.camera.resolution_y == 0) { *e = "zero-size image"; return false; }
+  if (s.world.kind == RD_WORLD_TRANSPARENT) {
+    *st = RD_ERR_UNSUPPORTED;
+    *e = "transparent world is not supported (raydar: todo!(\"transparent world support\"))";
+    return false;
+  }
+  if (s.world.kind > RD_WORLD_TRANSPARENT) { *e = "unknown world kind"; return false; }
+  if (d.samples == 0 || d.max_bounces == 0 || d.max_bounces > 255) {
+    *e = "samples must be >= 1 and max_bounces in 1..255";
+    return false;
+  }
+  if (s.object_count && !s.objects) { *e = "objects is NULL"; return false; }
+  for (uint32_t i = 0; i < s.object_count; ++i) {
+    const rd_object &o = s.objects[i];
+    if ((o.kind != RD_SPHERE && o.kind != RD_CUBE) || !std::isfinite(o.center[0]) ||
+        !std::isfinite(o.center[1]) || !std::isfinite(o.center[2]) || !std::isfinite(o.size)) {
+      *e = "object " + std::to_string(i) + ": unknown kind or non-finite geometry";
+      return false;
+    }
+  }
+  return true;
+}
+
 rd_status run_batch(rd_batch *b, const rd_job_desc *jobs, uint32_t n) {
   rdh::Streamer &st = b->st;
-  std::vector<uint64_t> ids;
-  const auto t0 = std::chrono::steady_clock::now();
-  const uint64_t it0 = st.iterations(), g0 = st.graph_launches, p0 = st.pumps, paths0 = st.paths_done;
   for (uint32_t i = 0; i < n; ++i) {
     std::string e;
-    auto j = st.make_job(jobs[i], st.eng.opt.leaf, &e);
-    if (!j) return batch_fail(b, e.find("transparent") != std::string::npos ? RD_ERR_UNSUPPORTED : RD_ERR_INVALID_ARGUMENT,
-                              "job " + std::to_string(i) + ": " + e);
-    j->id = b->next_id++;
-    ids.push_back(j->id);
-    st.enqueue(std::move(j));
+    rd_status s;
+    if (!validate_job(jobs[i], &e, &s)) return batch_fail(b, s, "job " + std::to_string(i) + ": " + e);
   }
-  while (!st.idle()) {
-    if (!st.pump()) return batch_fail(b, st.err.find("out of memory") != std::string::npos ? RD_ERR_OUT_OF_MEMORY : RD_ERR_DEVICE, st.err);
+  const auto t0 = std::chrono::steady_clock::now();
+  const uint64_t it0 = st.iterations(), g0 = st.graph_launches, p0 = st.pumps, paths0 = st.paths_done;
+  // Host-side scene digestion (BVH builds) runs on a thread pool and
+  // overlaps the GPU: jobs join the work list in submission order as soon as
+  // they are built.
+  std::vector<std::unique_ptr<rdh::SJob>> built(n);
+  std::vector<std::string> errs(n);
+  std::unique_ptr<std::atomic<int>[]> ready(new std::atomic<int>[n]);
+  for (uint32_t i = 0; i < n; ++i) ready[i] = 0;
+  std::atomic<uint32_t> next{0};
+  std::atomic<int64_t> build_ns{0};
+  const int leaf = st.eng.opt.leaf;
+  auto worker = [&] {
+    for (uint32_t i; (i = next.fetch_add(1)) < n;) {
+      const auto b0 = std::chrono::steady_clock::now();
+      built[i] = st.make_job(jobs[i], leaf, &errs[i]);
+      build_ns += rdh::ns_between(b0, std::chrono::steady_clock::now());
+      ready[i].store(built[i] ? 1 : 2, std::memory_order_release);
+    }
+  };
+  const uint32_t hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  std::vector<std::thread> pool;
+  for (uint32_t t = 0; t < std::min(hw, n); ++t) pool.emplace_back(worker);
+  std::vector<uint64_t> ids;
+  uint32_t enq = 0;
+  rd_status status = RD_OK;
+  while (status == RD_OK) {
+    while (enq < n && ready[enq].load(std::memory_order_acquire) != 0) {
+      if (ready[enq] == 2) {
+        status = batch_fail(b, RD_ERR_INVALID_ARGUMENT, "job " + std::to_string(enq) + ": " + errs[enq]);
+        break;
+      }
+      built[enq]->id = b->next_id++;
+      ids.push_back(built[enq]->id);
+      st.enqueue(std::move(built[enq]));
+      ++enq;
+    }
+    if (status != RD_OK) break;
+    if (st.idle()) {
+      if (enq == n) break;
+      std::this_thread::yield();  // waiting for the next scene to be built
+      continue;
+    }
+    if (!st.pump())
+      status = batch_fail(b, st.err.find("out of memory") != std::string::npos ? RD_ERR_OUT_OF_MEMORY : RD_ERR_DEVICE, st.err);
+  }
+  next = n;  // stop workers that have not started a job
+  for (auto &t : pool) t.join();
+  // after a failure, drain what was already submitted so no job is left half-rendered
+  while (status != RD_OK && !st.idle() && st.pump()) {
   }
   for (uint64_t id : ids) st.forget(id);
+  if (status != RD_OK) return status;
+  b->stats.build_ns = build_ns.load();
+  b->stats.host_threads = std::min(hw, n);
   b->stats.elapsed_ns = rdh::ns_between(t0, std::chrono::steady_clock::now());
   b->stats.iterations = st.iterations() - it0;
   b->stats.graph_launches = st.graph_launches - g0;
@@ -590,6 +666,8 @@ rd_status rd_batch_create(rd_batch **out) {
   *out = nullptr;
   auto *b = new (std::nothrow) rd_batch();
   if (!b) return RD_ERR_OUT_OF_MEMORY;
+  b->st.iters_per_pump = 4;  // throughput over latency: fewer host round trips
+  b->st.eng.opt.leaf = 2;    // measured fastest on mixed batches of small scenes
   if (!b->st.init()) {
     rdh::set_global_error(b->st.err);
     delete b;
@@ -779,6 +857,7 @@ rd_status rd_server_create(const rd_server_config *cfg, rd_server **out) {
     return RD_ERR_DEVICE;
   }
   s->st.default_pass = 8;
+  s->st.eng.opt.leaf = 2;
   if (cfg) {
     if (cfg->hbm_budget_bytes) s->st.budget = cfg->hbm_budget_bytes;
     if (cfg->pass_samples) s->st.default_pass = cfg->pass_samples;

@@ -25,28 +25,31 @@ namespace rdh {
 
 namespace {
 
+// Item boxes are the padded primitive boxes rounded outward to FP32, so every
+// node box (a union of item boxes) is exact in FP32 and still conservative.
 struct Item {
-  double lo[3], hi[3], c[3];
+  float lo[3], hi[3], c[3];
   uint32_t id;
 };
 
 struct Box {
-  double lo[3], hi[3];
+  float lo[3], hi[3];
   Box() {
     for (int a = 0; a < 3; ++a) {
-      lo[a] = std::numeric_limits<double>::infinity();
-      hi[a] = -std::numeric_limits<double>::infinity();
+      lo[a] = std::numeric_limits<float>::infinity();
+      hi[a] = -std::numeric_limits<float>::infinity();
     }
   }
-  void grow(const double *l, const double *h) {
+  void grow(const float *l, const float *h) {
     for (int a = 0; a < 3; ++a) {
       lo[a] = std::min(lo[a], l[a]);
       hi[a] = std::max(hi[a], h[a]);
     }
   }
+  void grow(const Box &b) { grow(b.lo, b.hi); }
   double area() const {
     double e[3];
-    for (int a = 0; a < 3; ++a) e[a] = std::max(0.0, hi[a] - lo[a]);
+    for (int a = 0; a < 3; ++a) e[a] = std::max(0.0, (double)hi[a] - (double)lo[a]);
     return 2.0 * (e[0] * e[1] + e[1] * e[2] + e[2] * e[0]);
   }
 };
@@ -59,6 +62,23 @@ double sphere_excess(double r, double L) {
   return q / (std::sqrt(r * r + q) + r);
 }
 
+float round_down(double v) {
+  float f = (float)v;
+  if ((double)f > v) f = std::nextafter(f, -std::numeric_limits<float>::infinity());
+  return f;
+}
+float round_up(double v) {
+  float f = (float)v;
+  if ((double)f < v) f = std::nextafter(f, std::numeric_limits<float>::infinity());
+  return f;
+}
+
+// Top-down binned SAH.  A node with at most leaf_size primitives becomes a
+// leaf; larger ones split at the cheapest of kBins-1 planes on each axis,
+// with a traversal cost of one primitive test per node visit.  All three
+// axes are binned in one pass over the node's items.  Near the stack-depth
+// limit the split falls back to the centroid median, which bounds the depth
+// by ceil(log2(n / leaf_size)) more levels.
 struct Builder {
   std::vector<Item> items;
   std::vector<DevNode> nodes;
@@ -77,14 +97,9 @@ struct Builder {
 
   // Returns the child code for items[b, e).
   int32_t build(size_t b, size_t e, uint32_t depth, const Box &bounds) {
-    const size_t n = e - b;
-    if (n <= (size_t)leaf_size) {
-      if (n == 1 || !worth_splitting(b, e, bounds)) return make_leaf(b, e, depth);
-    }
-    size_t mid = split(b, e, depth, bounds);
+    if (e - b <= (size_t)leaf_size) return make_leaf(b, e, depth);
     Box lb, rb;
-    for (size_t i = b; i < mid; ++i) lb.grow(items[i].lo, items[i].hi);
-    for (size_t i = mid; i < e; ++i) rb.grow(items[i].lo, items[i].hi);
+    const size_t mid = split(b, e, depth, bounds, &lb, &rb);
     const uint32_t idx = (uint32_t)nodes.size();
     nodes.emplace_back();
     max_depth_seen = std::max(max_depth_seen, depth);
@@ -100,20 +115,10 @@ struct Builder {
     return (int32_t)idx;
   }
 
-  static float down(double v) {
-    float f = (float)v;
-    if ((double)f > v) f = std::nextafter(f, -std::numeric_limits<float>::infinity());
-    return f;
-  }
-  static float up(double v) {
-    float f = (float)v;
-    if ((double)f < v) f = std::nextafter(f, std::numeric_limits<float>::infinity());
-    return f;
-  }
   static void set_box(DevNode &nd, int k, const Box &b) {
-    nd.lo_x[k] = down(b.lo[0]); nd.hi_x[k] = up(b.hi[0]);
-    nd.lo_y[k] = down(b.lo[1]); nd.hi_y[k] = up(b.hi[1]);
-    nd.lo_z[k] = down(b.lo[2]); nd.hi_z[k] = up(b.hi[2]);
+    nd.lo_x[k] = b.lo[0]; nd.hi_x[k] = b.hi[0];
+    nd.lo_y[k] = b.lo[1]; nd.hi_y[k] = b.hi[1];
+    nd.lo_z[k] = b.lo[2]; nd.hi_z[k] = b.hi[2];
   }
 
   int32_t make_leaf(size_t b, size_t e, uint32_t depth) {
@@ -123,85 +128,95 @@ struct Builder {
     return leaf_code(first, (uint32_t)(e - b));
   }
 
-  bool worth_splitting(size_t b, size_t e, const Box &bounds) {
-    // leaf cost n (one primitive test each) vs the best binned split,
-    // 1 + (A_l n_l + A_r n_r) / A (one node visit costs about one test)
-    double best;
-    best_split(b, e, bounds, &best, nullptr, nullptr);
-    return best < (double)(e - b);
-  }
+  struct Binning {
+    float lo[3], scale[3];
+    int bin(const Item &it, int a) const {
+      const int k = (int)((it.c[a] - lo[a]) * scale[a]);
+      return std::min(std::max(k, 0), kBins - 1);
+    }
+  };
 
-  // Best binned SAH split; returns cost, axis and bin boundary.
-  void best_split(size_t b, size_t e, const Box &bounds, double *cost, int *axis_out,
-                  double *plane_out) {
+  // Splits items[b, e) in place; returns the first index of the right half
+  // and the two halves' bounds.
+  size_t split(size_t b, size_t e, uint32_t depth, const Box &bounds, Box *lb, Box *rb) {
+    const size_t n = e - b;
     Box cb;
     for (size_t i = b; i < e; ++i) cb.grow(items[i].c, items[i].c);
-    *cost = std::numeric_limits<double>::infinity();
-    const double area = std::max(bounds.area(), 1e-30);
-    for (int a = 0; a < 3; ++a) {
-      const double ext = cb.hi[a] - cb.lo[a];
-      if (!(ext > 0.0)) continue;
-      Box bb[kBins];
-      size_t cnt[kBins] = {0};
-      for (size_t i = b; i < e; ++i) {
-        int k = (int)((items[i].c[a] - cb.lo[a]) / ext * kBins);
-        k = std::min(std::max(k, 0), kBins - 1);
-        cnt[k]++;
-        bb[k].grow(items[i].lo, items[i].hi);
+    const int need = ceil_log2((n + leaf_size - 1) / leaf_size);
+    size_t mid = b;
+    if ((int)depth + need + 1 < kMaxDepth) {
+      Binning bn;
+      bool any = false;
+      for (int a = 0; a < 3; ++a) {
+        const float ext = cb.hi[a] - cb.lo[a];
+        bn.lo[a] = cb.lo[a];
+        bn.scale[a] = ext > 0.f ? (float)kBins * (1.0f - 1e-6f) / ext : 0.f;
+        any |= ext > 0.f;
       }
-      Box left[kBins];
-      size_t lc[kBins];
-      Box acc;
-      size_t ac = 0;
-      for (int k = 0; k < kBins; ++k) {
-        acc.grow(bb[k].lo, bb[k].hi);
-        ac += cnt[k];
-        left[k] = acc;
-        lc[k] = ac;
-      }
-      Box racc;
-      size_t rc = 0;
-      for (int k = kBins - 1; k > 0; --k) {
-        racc.grow(bb[k].lo, bb[k].hi);
-        rc += cnt[k];
-        if (lc[k - 1] == 0 || rc == 0) continue;
-        const double c = 1.0 + (left[k - 1].area() * lc[k - 1] + racc.area() * rc) / area;
-        if (c < *cost) {
-          *cost = c;
-          if (axis_out) *axis_out = a;
-          if (plane_out) *plane_out = cb.lo[a] + ext * k / kBins;
+      int axis = -1, plane = 0;
+      Box bin_box[3][kBins];
+      uint32_t bin_n[3][kBins] = {};
+      if (any) {
+        for (size_t i = b; i < e; ++i) {
+          const Item &it = items[i];
+          for (int a = 0; a < 3; ++a) {
+            const int k = bn.bin(it, a);
+            bin_n[a][k]++;
+            bin_box[a][k].grow(it.lo, it.hi);
+          }
+        }
+        const double inv_area = 1.0 / std::max(bounds.area(), 1e-30);
+        double best = std::numeric_limits<double>::infinity();
+        for (int a = 0; a < 3; ++a) {
+          if (bn.scale[a] == 0.f) continue;
+          double left_cost[kBins];
+          Box acc;
+          uint32_t cnt = 0;
+          for (int k = 0; k < kBins - 1; ++k) {
+            acc.grow(bin_box[a][k]);
+            cnt += bin_n[a][k];
+            left_cost[k] = cnt ? acc.area() * cnt : -1.0;
+          }
+          acc = Box();
+          cnt = 0;
+          for (int k = kBins - 1; k > 0; --k) {
+            acc.grow(bin_box[a][k]);
+            cnt += bin_n[a][k];
+            if (cnt == 0 || left_cost[k - 1] < 0.0) continue;
+            const double c = 1.0 + (left_cost[k - 1] + acc.area() * cnt) * inv_area;
+            if (c < best) {
+              best = c;
+              axis = a;
+              plane = k;
+            }
+          }
         }
       }
+      if (axis >= 0) {
+        auto it = std::partition(items.begin() + b, items.begin() + e,
+                                 [&](const Item &x) { return bn.bin(x, axis) < plane; });
+        mid = (size_t)(it - items.begin());
+        *lb = Box();
+        *rb = Box();
+        for (int k = 0; k < plane; ++k) lb->grow(bin_box[axis][k]);
+        for (int k = plane; k < kBins; ++k) rb->grow(bin_box[axis][k]);
+      }
     }
-  }
-
-  size_t split(size_t b, size_t e, uint32_t depth, const Box &bounds) {
-    const size_t n = e - b;
-    const int need = ceil_log2((n + leaf_size - 1) / leaf_size);
-    bool median = (int)depth + need + 1 >= kMaxDepth;
-    int axis = -1;
-    double plane = 0.0, cost;
-    if (!median) {
-      best_split(b, e, bounds, &cost, &axis, &plane);
-      if (axis < 0) median = true;
+    if (mid == b || mid == e) {
+      // Median split on the widest centroid axis (also for coincident centroids).
+      int a = 0;
+      for (int k = 1; k < 3; ++k)
+        if (cb.hi[k] - cb.lo[k] > cb.hi[a] - cb.lo[a]) a = k;
+      mid = b + n / 2;
+      std::nth_element(items.begin() + b, items.begin() + mid, items.begin() + e,
+                       [&](const Item &x, const Item &y) {
+                         return x.c[a] < y.c[a] || (x.c[a] == y.c[a] && x.id < y.id);
+                       });
+      *lb = Box();
+      *rb = Box();
+      for (size_t i = b; i < mid; ++i) lb->grow(items[i].lo, items[i].hi);
+      for (size_t i = mid; i < e; ++i) rb->grow(items[i].lo, items[i].hi);
     }
-    if (!median) {
-      auto it = std::partition(items.begin() + b, items.begin() + e,
-                               [&](const Item &x) { return x.c[axis] < plane; });
-      size_t mid = (size_t)(it - items.begin());
-      if (mid > b && mid < e) return mid;
-    }
-    // Median split on the widest centroid axis (also for coincident centroids).
-    Box cb;
-    for (size_t i = b; i < e; ++i) cb.grow(items[i].c, items[i].c);
-    int a = 0;
-    for (int k = 1; k < 3; ++k)
-      if (cb.hi[k] - cb.lo[k] > cb.hi[a] - cb.lo[a]) a = k;
-    size_t mid = b + n / 2;
-    std::nth_element(items.begin() + b, items.begin() + mid, items.begin() + e,
-                     [&](const Item &x, const Item &y) {
-                       return x.c[a] < y.c[a] || (x.c[a] == y.c[a] && x.id < y.id);
-                     });
     return mid;
   }
 };
@@ -282,8 +297,8 @@ bool build_scene(const rd_scene &scene, int leaf_size, BuiltScene *out, std::str
     pad += 1e-6 * (r_all + l1(o.center) + size) + 1e-7;
     for (int a = 0; a < 3; ++a) {
       it.c[a] = o.center[a];
-      it.lo[a] = (double)o.center[a] - half - pad;
-      it.hi[a] = (double)o.center[a] + half + pad;
+      it.lo[a] = round_down((double)o.center[a] - half - pad);
+      it.hi[a] = round_up((double)o.center[a] + half + pad);
     }
   }
   if (n > 0) {

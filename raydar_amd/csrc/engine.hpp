@@ -36,7 +36,7 @@ struct EngineOptions {
   uint64_t l_capacity = 1ull << 27;  // radiance slots (16 B each)
   int grid = 0;                   // 0: occupancy-derived
   int brute_force = 0;
-  int leaf = 4;
+  int leaf = 1;                   // BVH leaf size; 1 measured fastest for one scene
   int count = 0;
   int kernel_timing = 0;
   int sync_every = 4;             // iterations between completion checks

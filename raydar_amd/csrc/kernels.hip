@@ -85,6 +85,37 @@ __device__ __forceinline__ uint32_t slot_of(const SegPlan &sp, uint32_t seg_cap,
   return s * seg_cap + (i - sp.pre[s]);
 }
 
+// Finished-path counts per ticket, gathered in LDS by one shade workgroup and
+// flushed to HBM once per launch (instead of one same-address atomic per wave).
+constexpr uint32_t kDoneEntries = 32;
+struct DoneTable {
+  uint32_t n;                       // entries claimed (may exceed kDoneEntries)
+  uint32_t ticket[kDoneEntries];
+  uint32_t count[kDoneEntries];
+};
+
+// A slot's ticket is written once, after the slot is claimed, so any match
+// found by a scan is genuine; two waves racing to insert the same ticket
+// merely claim two slots.  With the table full, the count goes straight to
+// HBM.
+__device__ __forceinline__ void done_add(DoneTable &d, uint32_t t, uint32_t c,
+                                         unsigned long long *global) {
+  volatile uint32_t *vt = d.ticket;
+  const uint32_t n = min(*(volatile uint32_t *)&d.n, kDoneEntries);
+  for (uint32_t k = 0; k < n; ++k)
+    if (vt[k] == t) {
+      atomicAdd(&d.count[k], c);
+      return;
+    }
+  const uint32_t k = atomicAdd(&d.n, 1u);
+  if (k < kDoneEntries) {
+    vt[k] = t;
+    atomicAdd(&d.count[k], c);
+  } else {
+    atomicAdd(&global[t], (unsigned long long)c);
+  }
+}
+
 // Queue of parity p.  Indexing a.q[p] with the runtime parity makes the
 // compiler fetch the pointer table through a vector address and keep all 28
 // pointers in VGPRs; a uniform select keeps them in scalar registers.
@@ -214,7 +245,16 @@ __global__ void k_generate(IterArgs a, int p) {
     const unsigned long long w = pl.cursor + i;
     const uint32_t s = segment_of(pl.pre_new, i);
     const uint32_t k = s * a.seg_cap + pl.cnt[s] + (i - pl.pre_new[s]);
-    const uint32_t ti = SINGLE ? 0u : find_ticket(a.tickets, t_lo, t_n, w);
+    uint32_t ti = 0;
+    if (!SINGLE) {
+      // the wave's lanes hold consecutive work indices: one search for the
+      // first active lane's index (uniform, scalar loads) serves nearly all
+      const unsigned long long w0 =
+          ((unsigned long long)__builtin_amdgcn_readfirstlane((uint32_t)(w >> 32)) << 32) |
+          __builtin_amdgcn_readfirstlane((uint32_t)w);
+      ti = find_ticket(a.tickets, t_lo, t_n, w0);
+      if (w >= a.tickets[ti].work_begin + a.tickets[ti].paths) ti = find_ticket(a.tickets, ti, t_n, w);
+    }
     const DevTicket &T = a.tickets[ti];
     const DevJob &J = a.jobs[SINGLE ? 0u : T.job];
     const uint32_t pidx = (uint32_t)(w - T.work_begin);
@@ -282,7 +322,13 @@ __global__ __launch_bounds__(BLOCK) void k_shade(IterArgs a, int p) {
   __shared__ uint32_t wave_off[kWaves];
   __shared__ uint32_t block_base;
   __shared__ SegPlan pl;
-  make_plan(a, p, pl);
+  __shared__ DoneTable done;
+  if (threadIdx.x == 0) done.n = 0;
+  if (threadIdx.x < kDoneEntries) {
+    done.ticket[threadIdx.x] = 0xffffffffu;
+    done.count[threadIdx.x] = 0;
+  }
+  make_plan(a, p, pl);  // its barriers also publish the table's reset
   const PathQueue q = queue_of(a, p), nq = queue_of(a, 1 - p);
   const uint32_t lane = lane_id();
   const uint32_t wid = threadIdx.x >> 6;
@@ -365,18 +411,22 @@ __global__ __launch_bounds__(BLOCK) void k_shade(IterArgs a, int p) {
     }
     if (finished) a.L[l_slot] = make_float4(nl.x, nl.y, nl.z, 1.0f);
     if (a.track_done) {
-      // finished paths per ticket: one atomic per distinct ticket per wave
+      // finished paths per ticket: each wave folds its lanes per distinct
+      // ticket into the block's LDS table, flushed to HBM once per launch
       uint64_t fm = __ballot(finished);
       while (fm) {
         const uint32_t leader = (uint32_t)(__ffsll((unsigned long long)fm) - 1);
         const uint32_t lt = (uint32_t)__builtin_amdgcn_readlane((int)ti, (int)leader);
         const uint64_t same = __ballot(finished && ti == lt);
-        if (lane == leader) atomicAdd(&a.ticket_done[lt], (unsigned long long)__popcll(same));
+        if (lane == leader) done_add(done, lt, (uint32_t)__popcll(same), a.ticket_done);
         fm &= ~same;
       }
     }
     __syncthreads();  // wave_off / block_base are rewritten next iteration
   }
+  // the loop ended with a barrier: every entry of the table is final
+  if (a.track_done && threadIdx.x < done.n && threadIdx.x < kDoneEntries && done.count[threadIdx.x])
+    atomicAdd(&a.ticket_done[done.ticket[threadIdx.x]], (unsigned long long)done.count[threadIdx.x]);
   if (COUNT) {
     if (c_sph) atomicAdd(&a.counters->v[C_SPHERE_HIT], (unsigned long long)c_sph);
     if (c_cube) atomicAdd(&a.counters->v[C_CUBE_HIT], (unsigned long long)c_cube);

@@ -130,6 +130,15 @@ def test_server_rejects_invalid_jobs():
         assert srv.stats()["jobs_rejected"] == 2
 
 
+def test_poisson_serving_run_completes():
+    from raydar_amd import serving
+    arrivals = scenegen.serve_arrivals(rate=20.0, duration=1.5)
+    rep = serving.run(arrivals, hbm_budget_bytes=200 << 30, pass_samples=8)
+    print(rep)
+    assert rep["all_ok"] and rep["requests"] == len(arrivals)
+    assert 0 < rep["p50_ms"] <= rep["p99_ms"] <= rep["max_ms"]
+
+
 def test_autotune_returns_a_valid_setting():
     jobs = [Job(scenegen.plain_small(0), 1, 4, 0), Job(scenegen.clustered(5, 200, 64, 64), 1, 4, 1)]
     with BatchRenderer() as br:

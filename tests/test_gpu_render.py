@@ -157,6 +157,45 @@ def test_transparent_world_is_an_error():
         assert e.value.status == abi.RD_ERR_UNSUPPORTED
 
 
+def _read_png(path):
+    import struct
+    import zlib
+    data = open(path, "rb").read()
+    assert data[:8] == b"\x89PNG\r\n\x1a\n"
+    pos, idat = 8, b""
+    w = h = None
+    while pos < len(data):
+        n, = struct.unpack(">I", data[pos:pos + 4])
+        kind, body = data[pos + 4:pos + 8], data[pos + 8:pos + 8 + n]
+        if kind == b"IHDR":
+            w, h = struct.unpack(">II", body[:8])
+            assert body[8:10] == b"\x08\x06"
+        elif kind == b"IDAT":
+            idat += body
+        pos += 12 + n
+    raw = zlib.decompress(idat)
+    rows = [raw[y * (4 * w + 1):(y + 1) * (4 * w + 1)] for y in range(h)]
+    assert all(r[0] == 0 for r in rows)  # filter type None
+    return np.frombuffer(b"".join(r[1:] for r in rows), np.uint8).reshape(h, w, 4)
+
+
+def test_cli_renders_scene_file(tmp_path):
+    """raydar_hip (main.rs counterpart): flags, .rscn input, PNG output."""
+    import subprocess
+    root = os.path.dirname(GOLDEN)
+    cli = os.path.join(os.path.dirname(root), "raydar_amd", "raydar_hip")
+    out = str(tmp_path / "out.png")
+    p = subprocess.run([cli, "--max-sample-count", "3", "--max-bounces", "5", "-o", out,
+                        os.path.join(GOLDEN, "default.rscn")], capture_output=True, text=True,
+                       timeout=120)
+    assert p.returncode == 0, p.stderr
+    assert "Max Samples: 3" in p.stdout and "Total Frame Time:" in p.stdout
+    img = _read_png(out)
+    sc = Scene.load(os.path.join(GOLDEN, "default.rscn"))
+    ref = oracle.tonemap(oracle.render(sc, 3, 5, 0), 3).reshape(sc.height, sc.width, 4)
+    np.testing.assert_array_equal(img, ref)
+
+
 def test_resolution_change_reallocates():
     a = scenegen.plain_small(0)
     b = scenegen.plain_small(0)
