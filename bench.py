@@ -138,6 +138,39 @@ def serving(seconds=60.0, rate=20.0):
     return rep
 
 
+def calibrate_copy(nbytes):
+    """The library's plain copy kernel (k_copy_calibration: read nbytes, write
+    nbytes) five times: what FETCH_SIZE / WRITE_SIZE read for a known amount
+    of HBM traffic."""
+    import raydar_amd as ra
+    ms = ra.debug_copy_calibration(nbytes, 5)
+    print(json.dumps({"calibrate_copy_bytes": nbytes, "launches": 5,
+                      "ms": [round(float(x), 4) for x in ms],
+                      "GB_per_s": round(2 * nbytes / (float(np.median(ms)) * 1e-3) / 1e9, 1)}),
+          flush=True)
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "extend_plain_1080p_hbm_traffic.md")
+
+
+def extend_traffic(workload, paths_per_step, spp):
+    """HBM bytes per extend launch from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE + WRITE_SIZE, each scaled by the copy-kernel calibration).
+    The counters cannot be read from inside this process, so the figure is
+    the one measured for this workload by the command the summary names."""
+    if workload != "plain_1080p" or not os.path.exists(TRAFFIC_FILE):
+        return None
+    vals = {}
+    with open(TRAFFIC_FILE) as f:
+        for line in f:
+            if line.startswith("traffic."):
+                k, v = line[len("traffic."):].split("=", 1)
+                vals[k.strip()] = float(v)
+    if "bytes_per_launch" not in vals:
+        return None
+    return round(vals["bytes_per_launch"])
+
+
 def _cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -163,11 +196,19 @@ def main():
                     help="length of the Poisson serving run (0 skips it)")
     ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--option", action="append", default=[], help="key=value renderer option")
+    ap.add_argument("--timed-only", action="store_true",
+                    help="only the warmup and timed steps (for runs under a profiler)")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
+                    help="run a plain device copy of BYTES bytes 5 times and exit "
+                         "(the reference point for HBM counter readings)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
               "(this build renders on a single MI355X)", file=sys.stderr)
         sys.exit(2)
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy)
+        return
 
     import raydar_amd as ra
     from raydar_amd import scenegen
@@ -197,6 +238,12 @@ def main():
     value = paths_per_step * args.steps / elapsed / 1e6
     print(f"# timed {args.steps} steps: {ms_per_step:.2f} ms/step, {value:.1f} Mpaths/s",
           file=sys.stderr, flush=True)
+    if args.timed_only:
+        print(json.dumps({"metric": METRIC, "value": round(value, 3), "unit": "Mpaths/s",
+                          "ms_per_step": round(ms_per_step, 3), "steps": args.steps,
+                          "warmup": args.warmup, "timed_only": True}), flush=True)
+        r.close()
+        return
 
     # counting pass (algorithmic events)
     r.reset_counters()
@@ -216,6 +263,7 @@ def main():
     ext_avg_s = ext_ms / 1e3 / max(ext_n, 1)
     achieved = ext_flop_per_launch / ext_avg_s / 1e12 if ext_avg_s > 0 else 0.0
     rays_per_step = cnt["segments"]
+    traffic = extend_traffic(args.workload, paths_per_step, job.spp)
     total_flop = path_flops(cnt)
     step_s = ms_per_step / 1e3
 
@@ -253,7 +301,7 @@ def main():
             "unit": "TFLOP/s",
             "frac": round(achieved / PEAK_FP32_TFLOPS, 6),
             "frac_of_issue_ceiling": round(achieved / ISSUE_TOPS, 6),
-            "traffic": None,
+            "traffic": traffic,
             "flop_per_launch": round(ext_flop_per_launch),
             "avg_launch_ms": round(ext_avg_s * 1e3, 5),
             "launches": ext_n,

@@ -47,6 +47,13 @@ rd_status rd_debug_world(const rd_scene *scene, const float *dirs, uint32_t n, f
 /* Philox-4x32-10 on the GPU: in n x 6 words (counter[4], key[2]), out n x 4. */
 rd_status rd_debug_philox(const uint32_t *in, uint32_t n, uint32_t *out);
 
+/* A plain device copy of `bytes` (multiple of 16, <= 32 GiB) between two
+ * fresh buffers, `reps` launches of k_copy_calibration, each timed with
+ * hipEvents into ms[reps] (may be NULL).  It reads and writes exactly
+ * `bytes` of HBM per launch: the reference point that converts hardware
+ * traffic counters (FETCH_SIZE / WRITE_SIZE) into bytes. */
+rd_status rd_debug_copy_calibration(uint64_t bytes, uint32_t reps, float *ms);
+
 /* Host: build the BVH and report node count, depth and whether its
  * structural invariants hold (message in err when they do not). */
 rd_status rd_debug_host_bvh_info(const rd_scene *scene, int leaf_size, uint32_t *n_nodes,

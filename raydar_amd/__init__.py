@@ -69,6 +69,7 @@ def lib():
         "rd_debug_camera_rays": (C.c_int, [sp, C.c_void_p, C.c_uint32, C.c_void_p]),
         "rd_debug_world": (C.c_int, [sp, C.c_void_p, C.c_uint32, C.c_void_p]),
         "rd_debug_philox": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p]),
+        "rd_debug_copy_calibration": (C.c_int, [C.c_uint64, C.c_uint32, C.c_void_p]),
         "rd_debug_host_bvh_info": (C.c_int, [sp, C.c_int, P(C.c_uint32), P(C.c_uint32),
                                              C.c_char_p, C.c_size_t]),
         "rd_debug_host_trace": (C.c_int, [sp, C.c_int, C.c_void_p, C.c_uint32, C.c_void_p,
@@ -271,6 +272,14 @@ def debug_world(scene, dirs):
     s = scene.as_rd()
     check(lib().rd_debug_world(C.byref(s), _ptr(dirs), len(dirs), _ptr(out)))
     return out
+
+
+def debug_copy_calibration(nbytes, reps=5):
+    """Runs the library's plain copy kernel `reps` times over nbytes; returns
+    each launch's duration in ms."""
+    ms = np.zeros(reps, np.float32)
+    check(lib().rd_debug_copy_calibration(int(nbytes), int(reps), _ptr(ms)))
+    return ms
 
 
 def debug_philox(ctr_key):

@@ -619,6 +619,41 @@ rd_status rd_debug_philox(const uint32_t *in, uint32_t n, uint32_t *out) {
   return finish(hipSuccess);
 }
 
+rd_status rd_debug_copy_calibration(uint64_t bytes, uint32_t reps, float *ms) {
+  if (bytes == 0 || bytes % 16 || bytes > (1ull << 35) || reps == 0 || reps > 1000)
+    return RD_ERR_INVALID_ARGUMENT;
+  int nd = 0;
+  if (hipGetDeviceCount(&nd) != hipSuccess || nd == 0) {
+    (void)hipGetLastError();
+    rdh::set_global_error("no HIP device available");
+    return RD_ERR_DEVICE;
+  }
+  DebugCtx c;
+  uint8_t *src = c.dev<uint8_t>(nullptr, bytes), *dst = c.dev<uint8_t>(nullptr, bytes);
+  if (!src || !dst) return RD_ERR_OUT_OF_MEMORY;
+  if (hipMemset(src, 1, bytes) != hipSuccess || hipDeviceSynchronize() != hipSuccess)
+    return finish(hipGetLastError());
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess) return finish(hipGetLastError());
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return finish(hipGetLastError());
+  }
+  hipError_t err = hipSuccess;
+  for (uint32_t r = 0; r < reps && err == hipSuccess; ++r) {
+    (void)hipEventRecord(e0, nullptr);
+    err = rdh::launch_copy_calibration(src, dst, bytes, nullptr);
+    (void)hipEventRecord(e1, nullptr);
+    if (err == hipSuccess) err = hipEventSynchronize(e1);
+    float t = 0.f;
+    if (err == hipSuccess) err = hipEventElapsedTime(&t, e0, e1);
+    if (ms) ms[r] = t;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return finish(err);
+}
+
 rd_status rd_debug_host_bvh_info(const rd_scene *scene, int leaf, uint32_t *n_nodes,
                                  uint32_t *depth, char *err, size_t cap) {
   if (!scene || !n_nodes || !depth) return RD_ERR_INVALID_ARGUMENT;

@@ -583,6 +583,12 @@ __global__ void k_debug_philox(const uint32_t *in, uint32_t n, uint32_t *out) {
   for (int k = 0; k < 4; ++k) out[4 * (uint64_t)i + k] = r.v[k];
 }
 
+// Plain 16-byte-per-lane copy: the reference point for HBM traffic counters.
+__global__ void k_copy_calibration(const uint4 *src, uint4 *dst, uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
+
 uint32_t blocks_for(uint64_t n, uint32_t block) { return (uint32_t)((n + block - 1) / block); }
 
 }  // namespace
@@ -719,6 +725,14 @@ hipError_t launch_debug_world(const DevJob *job, const float *dirs, uint32_t n, 
                               hipStream_t st) {
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_debug_world, dim3(blocks_for(n, 256)), dim3(256), 0, st, job, dirs, n, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_copy_calibration(const void *src, void *dst, uint64_t bytes, hipStream_t st) {
+  const uint64_t n = bytes / 16;
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_copy_calibration, dim3(blocks_for(n, 256)), dim3(256), 0, st,
+                     static_cast<const uint4 *>(src), static_cast<uint4 *>(dst), n);
   return hipGetLastError();
 }
 

@@ -74,6 +74,8 @@ hipError_t launch_debug_camera(const DevJob *job, const uint32_t *pixels, uint32
                                hipStream_t st);
 hipError_t launch_debug_world(const DevJob *job, const float *dirs, uint32_t n, float *out,
                               hipStream_t st);
+// dst[0, bytes) = src[0, bytes); bytes a multiple of 16, at most 2^35.
+hipError_t launch_copy_calibration(const void *src, void *dst, uint64_t bytes, hipStream_t st);
 hipError_t launch_debug_philox(const uint32_t *in, uint32_t n, uint32_t *out, hipStream_t st);
 
 }  // namespace rdh

@@ -206,3 +206,12 @@ def test_resolution_change_reallocates():
         out = r.render_sample(b)
         assert out.shape == (64, 96, 4)
         assert r.sample_count == 2
+
+
+def test_copy_calibration_kernel():
+    # the HBM-counter reference point used by the traffic profile
+    ms = ra.debug_copy_calibration(1 << 26, 3)
+    assert ms.shape == (3,) and (ms > 0).all()
+    with pytest.raises(ra.RaydarError) as e:
+        ra.debug_copy_calibration(1000, 1)  # not a multiple of 16
+    assert e.value.status == abi.RD_ERR_INVALID_ARGUMENT
