@@ -196,6 +196,9 @@ rd_status rd_read_accumulator(rd_renderer *r, float *out, size_t float_capacity)
  *   "leaf"        BVH leaf size 1..8, default 1 (takes effect at the next new_frame)
  *   "grid"        workgroups per persistent launch (0 = automatic)
  *   "traversal"   0 = BVH, 1 = brute-force linear scan (cpu.rs:345-349)
+ *   "refill"      closest-hit lanes: 0 = each lane traces its rays to the end
+ *                 in launch order; k in 1..64 = a wave hands new rays to its
+ *                 idle lanes once k or more are idle (default 48)
  *   "count"       1 = collect algorithmic event counters in the next passes */
 rd_status rd_set_option(rd_renderer *r, const char *key, int64_t value);
 rd_status rd_get_option(const rd_renderer *r, const char *key, int64_t *value);

@@ -53,7 +53,8 @@ typedef struct rd_batch_stats {
 rd_status rd_batch_create(rd_batch **out);
 void rd_batch_destroy(rd_batch *b);
 /* Keys: "block" (64/128/256), "queue" (paths in flight), "leaf" (1..8, default 2),
- * "grid" (workgroups, 0 = auto), "graph" (1 = replay a captured hipGraph),
+ * "grid" (workgroups, 0 = auto), "refill" (0..64, as rd_set_option),
+ * "graph" (1 = replay a captured hipGraph),
  * "l_capacity" (radiance slots), "iters_per_pump" (even, >= 2). */
 rd_status rd_batch_set_option(rd_batch *b, const char *key, int64_t value);
 rd_status rd_batch_get_option(const rd_batch *b, const char *key, int64_t *value);

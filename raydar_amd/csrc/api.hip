@@ -217,6 +217,7 @@ bool option_value(const rd_renderer *r, const std::string &k, int64_t *v) {
   else if (k == "leaf") *v = o.leaf;
   else if (k == "grid") *v = o.grid;
   else if (k == "traversal") *v = o.brute_force;
+  else if (k == "refill") *v = o.refill;
   else if (k == "count") *v = o.count;
   else if (k == "kernel_timing") *v = o.kernel_timing;
   else if (k == "l_capacity") *v = (int64_t)o.l_capacity;
@@ -398,6 +399,9 @@ rd_status rd_set_option(rd_renderer *r, const char *key, int64_t v) {
   } else if (k == "traversal") {
     if (v != 0 && v != 1) return r->fail(RD_ERR_INVALID_ARGUMENT, "traversal must be 0 or 1");
     o.brute_force = (int)v;
+  } else if (k == "refill") {
+    if (v < 0 || v > 64) return r->fail(RD_ERR_INVALID_ARGUMENT, "refill must be in 0..64");
+    o.refill = (int)v;
   } else if (k == "count") {
     o.count = v != 0;
   } else if (k == "kernel_timing") {

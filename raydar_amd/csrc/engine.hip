@@ -187,6 +187,7 @@ IterArgs Engine::iter_args(const DevJob *d_jobs) const {
   a.ticket_done = d_ticket_done;
   a.counters = counters;
   a.brute_force = opt.brute_force;
+  a.refill = opt.refill;
   a.single_job = 0;
   a.track_done = 1;
   a.wide_stack = 0;
@@ -199,7 +200,7 @@ LaunchShape Engine::shape(int which) const {
   if (opt.grid > 0) {
     s.grid = opt.grid;
   } else {
-    const int per_cu = std::max(1, occupancy_blocks(which, s.block));
+    const int per_cu = std::max(1, occupancy_blocks(which, s.block, which == 1 && opt.refill > 0));
     s.grid = num_cus * per_cu;
   }
   return s;

@@ -36,6 +36,7 @@ struct EngineOptions {
   uint64_t l_capacity = 1ull << 27;  // radiance slots (16 B each)
   int grid = 0;                   // 0: occupancy-derived
   int brute_force = 0;
+  int refill = 48;                // extend: persistent lanes refill at >= 48 idle (0: off)
   int leaf = 1;                   // BVH leaf size; 1 measured fastest for one scene
   int count = 0;
   int kernel_timing = 0;

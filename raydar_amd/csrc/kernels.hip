@@ -309,6 +309,143 @@ void k_extend(IterArgs a, int p) {
   }
 }
 
+// Persistent-lane closest hit.  Rays differ widely in traversal length, so
+// a wave that traces 64 rays to completion keeps most lanes idle while its
+// longest ray finishes.  Here each wave owns a contiguous range of queue
+// indices (no shared counter: handing out chunks through one device-wide
+// atomic measured twice as slow on short rays),
+// and a lane that finishes takes the next index of its wave's range as soon
+// as `refill` or more lanes of the wave are idle.  Each step of the loop
+// visits one BVH node per active lane.  Indices only grow per lane, so
+// slot_of's running segment stays valid.
+template <int BLOCK, bool COUNT, bool SINGLE, class StackT>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(RDH_EXTEND_WAVES, 8)))
+void k_extend_lanes(IterArgs a, int p) {
+  __shared__ StackT stack[kStackDepth * BLOCK];
+  __shared__ SegPlan pl;
+  make_plan(a, p, pl);
+  // queue[1-p] is empty until shade fills it; nothing else reads it now
+  if (blockIdx.x == 0 && threadIdx.x < kSegments) a.ctl->count[1 - p][threadIdx.x * kSegStride] = 0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) a.ctl->cursor[1 - p] = pl.cursor + pl.n_new;
+  const PathQueue q = queue_of(a, p);
+  const uint32_t n = pl.n;
+  const uint32_t waves = gridDim.x * (BLOCK / 64);
+  const uint32_t wave = blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6);
+  uint32_t c_next = (uint32_t)((uint64_t)n * wave / waves);
+  const uint32_t c_end = (uint32_t)((uint64_t)n * (wave + 1) / waves);
+  bool more = c_next < c_end;  // wave-uniform: the wave still has rays to hand out
+  StackT *stk = stack + threadIdx.x;
+  const uint32_t refill = (uint32_t)a.refill;
+
+  const DevJob &J0 = a.jobs[0];
+  const DevNode *nodes = J0.nodes;
+  const DevPrim *prims = J0.prims;
+  bool active = false;
+  uint32_t k = 0, seg = 0, bid = kNoHit;
+  int32_t node = 0;
+  int sp = 0;
+  float bt = 0.0f;
+  RayPre r;
+  SlabRay s;
+  TraceCounts tc;
+  uint32_t rays = 0;
+  for (;;) {
+    const uint64_t idle = __ballot(!active);
+    if (more && (__popcll(idle) >= refill || idle == ~0ull)) {
+      // hand the wave's next indices to its idle lanes, in lane order
+      const uint32_t take = min((uint32_t)__popcll(idle), c_end - c_next);
+      const uint32_t rank = lanes_below(idle);
+      const bool got = !active && rank < take;
+      const uint32_t mine = c_next + rank;
+      c_next += take;
+      more = c_next < c_end;
+      if (got) {
+        k = slot_of(pl, a.seg_cap, mine, seg);
+        const F3 o = f3(q.ox[k], q.oy[k], q.oz[k]);
+        const F3 d = f3(q.dx[k], q.dy[k], q.dz[k]);
+        const DevJob &J = a.jobs[SINGLE ? 0u : a.tickets[q.tb[k] >> 8].job];
+        r = ray_pre(o, d);
+        bt = 0.0f;
+        bid = kNoHit;
+        const bool finite = isfinite(r.ox) && isfinite(r.oy) && isfinite(r.oz) && isfinite(r.dx) &&
+                            isfinite(r.dy) && isfinite(r.dz);
+        if (J.n_prims == 0 || a.brute_force || !finite || J.n_nodes == 0) {
+          // the reference's linear scan (cpu.rs:345-349), also the exact
+          // answer for non-finite rays; done at once
+          for (uint32_t j = 0; j < J.n_prims; ++j) {
+            DevPrim pr;
+            load_prim(J.prims, j, &pr);
+            if (COUNT) {
+              if (fbits(pr.b[3]) & kTagCube) tc.cube++;
+              else tc.sphere++;
+            }
+            test_prim(r, pr, &bt, &bid);
+          }
+          a.hit_t[k] = bid == kNoHit ? 0.0f : bt;
+          a.hit_id[k] = bid;
+          ++rays;
+        } else {
+          if (!SINGLE) {
+            nodes = J.nodes;
+            prims = J.prims;
+          }
+          s = slab_ray(r, ray_tolerance(r, J.r0_tol));
+          node = 0;
+          sp = 0;
+          active = true;
+        }
+      }
+    }
+    if (__ballot(active) == 0) {
+      if (!more) break;
+      continue;
+    }
+    if (active) {
+      // one node: both child boxes, leaves tested at once
+      const float4 *np = reinterpret_cast<const float4 *>(nodes + node);
+      const float4 bx = np[0], by = np[1], bz = np[2];
+      const int4 ch = reinterpret_cast<const int4 *>(np)[3];
+      const float tcull = cull_limit(bt, bid);
+      float t0, t1;
+      bool h0 = slab_hit(s, bx.x, bx.z, by.x, by.z, bz.x, bz.z, tcull, &t0);
+      bool h1 = slab_hit(s, bx.y, bx.w, by.y, by.w, bz.y, bz.w, tcull, &t1);
+      if (COUNT) tc.box += 2;
+      if (h0 && is_leaf(ch.x)) {
+        test_leaf<COUNT>(r, prims, ch.x, &bt, &bid, tc);
+        h0 = false;
+      }
+      if (h1 && is_leaf(ch.y)) {
+        test_leaf<COUNT>(r, prims, ch.y, &bt, &bid, tc);
+        h1 = false;
+      }
+      if (h0 && h1) {
+        const bool first0 = t0 <= t1;
+        stk[sp * BLOCK] = (StackT)(first0 ? ch.y : ch.x);
+        ++sp;
+        node = first0 ? ch.x : ch.y;
+      } else if (h0) {
+        node = ch.x;
+      } else if (h1) {
+        node = ch.y;
+      } else if (sp > 0) {
+        --sp;
+        node = (int32_t)stk[sp * BLOCK];
+      } else {
+        a.hit_t[k] = bid == kNoHit ? 0.0f : bt;
+        a.hit_id[k] = bid;
+        active = false;
+        ++rays;
+      }
+    }
+  }
+  if (COUNT) {
+    if (rays) atomicAdd(&a.counters->v[C_SEGMENTS], (unsigned long long)rays);
+    if (tc.box) atomicAdd(&a.counters->v[C_BOX], (unsigned long long)tc.box);
+    if (tc.sphere) atomicAdd(&a.counters->v[C_SPHERE], (unsigned long long)tc.sphere);
+    if (tc.cube) atomicAdd(&a.counters->v[C_CUBE], (unsigned long long)tc.cube);
+  }
+}
+
 // --------------------------------------------------------------- shade ----
 // Survivors are compacted per workgroup: a ballot + mbcnt gives each lane its
 // slot inside its wave, an LDS scan over the block's waves gives each wave its
@@ -597,11 +734,15 @@ uint32_t blocks_for(uint64_t n, uint32_t block) { return (uint32_t)((n + block -
 // Kernel instance for a launch: block size, counting build, single-job
 // specialisation and stack width are template parameters.
 template <bool COUNT, bool SINGLE>
-static const void *extend_fn(int block, bool wide) {
+static const void *extend_fn(int block, bool wide, bool lanes) {
 #define RDH_EXT(B)                                                               \
-  if (block == B)                                                                \
+  if (block == B) {                                                              \
+    if (lanes)                                                                   \
+      return wide ? (const void *)k_extend_lanes<B, COUNT, SINGLE, int32_t>      \
+                  : (const void *)k_extend_lanes<B, COUNT, SINGLE, uint16_t>;    \
     return wide ? (const void *)k_extend<B, COUNT, SINGLE, int32_t>              \
-                : (const void *)k_extend<B, COUNT, SINGLE, uint16_t>;
+                : (const void *)k_extend<B, COUNT, SINGLE, uint16_t>;            \
+  }
   RDH_EXT(64)
   RDH_EXT(128)
   RDH_EXT(256)
@@ -615,21 +756,23 @@ static const void *shade_fn(int block) {
   if (block == 256) return (const void *)k_shade<256, COUNT, SINGLE>;
   return nullptr;
 }
-static const void *kernel_fn(int which, int block, bool count, bool single, bool wide) {
+static const void *kernel_fn(int which, int block, bool count, bool single, bool wide,
+                             bool lanes = false) {
   if (which == 0) {
     if (count) return single ? (const void *)k_generate<true, true> : (const void *)k_generate<true, false>;
     return single ? (const void *)k_generate<false, true> : (const void *)k_generate<false, false>;
   }
   if (which == 1) {
-    if (count) return single ? extend_fn<true, true>(block, wide) : extend_fn<true, false>(block, wide);
-    return single ? extend_fn<false, true>(block, wide) : extend_fn<false, false>(block, wide);
+    if (count)
+      return single ? extend_fn<true, true>(block, wide, lanes) : extend_fn<true, false>(block, wide, lanes);
+    return single ? extend_fn<false, true>(block, wide, lanes) : extend_fn<false, false>(block, wide, lanes);
   }
   if (count) return single ? shade_fn<true, true>(block) : shade_fn<true, false>(block);
   return single ? shade_fn<false, true>(block) : shade_fn<false, false>(block);
 }
 
-int occupancy_blocks(int which, int block) {
-  const void *fn = kernel_fn(which, block, false, true, false);
+int occupancy_blocks(int which, int block, bool lanes) {
+  const void *fn = kernel_fn(which, block, false, true, false, lanes);
   int n = 0;
   if (!fn || hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, block, 0) != hipSuccess) {
     (void)hipGetLastError();
@@ -640,7 +783,8 @@ int occupancy_blocks(int which, int block) {
 
 static hipError_t launch_stage(int which, const IterArgs &a, int parity, const LaunchShape &s,
                                bool count, hipStream_t st) {
-  const void *fn = kernel_fn(which, s.block, count, a.single_job != 0, a.wide_stack != 0);
+  const void *fn = kernel_fn(which, s.block, count, a.single_job != 0, a.wide_stack != 0,
+                             a.refill > 0);
   if (!fn) return hipErrorInvalidValue;
   IterArgs args = a;
   int par = parity;

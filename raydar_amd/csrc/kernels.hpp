@@ -37,6 +37,9 @@ struct IterArgs {
   int single_job;                   // 1: every path belongs to job 0 / ticket 0
   int track_done;                   // 1: count finished paths per ticket
   int wide_stack;                   // 1: some BVH has more than 65535 nodes
+  int refill;                       // extend: 0 = one ray per lane per pass over the
+                                    // queue; k > 0 = persistent lanes, a wave takes new
+                                    // rays once k or more of its lanes are idle
 };
 
 struct LaunchShape {
@@ -55,7 +58,7 @@ hipError_t launch_accumulate(const DevJob *job, const DevTicket &t, const float4
 hipError_t launch_tonemap(const DevJob *job, uint32_t n_pixels, float spp, uchar4 *out,
                           DevCounters *counters, bool count, int grid, hipStream_t st);
 // Workgroups of kernel `which` (0 generate, 1 extend, 2 shade) that fit on one CU.
-int occupancy_blocks(int which, int block);
+int occupancy_blocks(int which, int block, bool lanes = false);
 
 hipError_t launch_set_ctl(QueueCtl *ctl, unsigned long long total, uint32_t n_tickets,
                           hipStream_t st);

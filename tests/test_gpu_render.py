@@ -37,7 +37,9 @@ def _gpu_acc(scene, spp, bounces, seed, **opts):
 
 
 @pytest.mark.parametrize("opts", [dict(), dict(leaf=1, block=64, queue=4096),
-                                  dict(leaf=8, block=128, queue=65536), dict(traversal=1)])
+                                  dict(leaf=8, block=128, queue=65536), dict(traversal=1),
+                                  dict(refill=0), dict(refill=1, grid=7), dict(refill=64, leaf=2),
+                                  dict(refill=0, block=64, queue=4096)])
 def test_plain_small_matches_oracle(opts):
     sc = scenegen.plain_small(0)
     img, acc = _gpu_acc(sc, 16, 12, 5, **opts)
