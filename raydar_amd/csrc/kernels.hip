@@ -136,11 +136,31 @@ __device__ __forceinline__ uint32_t find_ticket(const DevTicket *t, uint32_t lo,
   return lo;
 }
 
+// Scene buffers are reached through pointers read from the job table, which
+// the compiler cannot place in an address space; these loads name the global
+// space explicitly so they become global_load (not flat_load, which also
+// waits on the LDS counter the traversal stack uses).
+typedef float VecF4 __attribute__((ext_vector_type(4)));
+typedef int VecI4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(1))) const VecF4 GlobalF4;
+typedef __attribute__((address_space(1))) const VecI4 GlobalI4;
+
 __device__ __forceinline__ void load_prim(const DevPrim *prims, uint32_t j, DevPrim *p) {
-  const float4 *s = reinterpret_cast<const float4 *>(prims + j);
-  const float4 a = s[0], b = s[1];
+  GlobalF4 *s = (GlobalF4 *)(prims + j);
+  const VecF4 a = s[0], b = s[1];
   p->a[0] = a.x; p->a[1] = a.y; p->a[2] = a.z; p->a[3] = a.w;
   p->b[0] = b.x; p->b[1] = b.y; p->b[2] = b.z; p->b[3] = b.w;
+}
+
+__device__ __forceinline__ DevObject load_object(const DevObject *objects, uint32_t id) {
+  GlobalF4 *s = (GlobalF4 *)(objects + id);
+  DevObject o;
+  const VecF4 g = s[0], m0 = s[1], m1 = s[2], m2 = s[3];
+  o.geo[0] = g.x; o.geo[1] = g.y; o.geo[2] = g.z; o.geo[3] = g.w;
+  o.m0[0] = m0.x; o.m0[1] = m0.y; o.m0[2] = m0.z; o.m0[3] = m0.w;
+  o.m1[0] = m1.x; o.m1[1] = m1.y; o.m1[2] = m1.z; o.m1[3] = m1.w;
+  o.m2[0] = m2.x; o.m2[1] = m2.y; o.m2[2] = m2.z; o.m2[3] = m2.w;
+  return o;
 }
 
 struct TraceCounts {
@@ -401,23 +421,32 @@ void k_extend_lanes(IterArgs a, int p) {
       continue;
     }
     if (active) {
-      // one node: both child boxes, leaves tested at once
-      const float4 *np = reinterpret_cast<const float4 *>(nodes + node);
-      const float4 bx = np[0], by = np[1], bz = np[2];
-      const int4 ch = reinterpret_cast<const int4 *>(np)[3];
+      // one node: both child boxes, then the primitives of the hit leaf
+      // children in one loop (lanes with a leaf on either side run it
+      // together; the closest hit does not depend on the test order)
+      GlobalF4 *np = (GlobalF4 *)(nodes + node);
+      const VecF4 bx = np[0], by = np[1], bz = np[2];
+      const VecI4 ch = ((GlobalI4 *)np)[3];
       const float tcull = cull_limit(bt, bid);
       float t0, t1;
       bool h0 = slab_hit(s, bx.x, bx.z, by.x, by.z, bz.x, bz.z, tcull, &t0);
       bool h1 = slab_hit(s, bx.y, bx.w, by.y, by.w, bz.y, bz.w, tcull, &t1);
       if (COUNT) tc.box += 2;
-      if (h0 && is_leaf(ch.x)) {
-        test_leaf<COUNT>(r, prims, ch.x, &bt, &bid, tc);
-        h0 = false;
+      const bool l0 = h0 && is_leaf(ch.x), l1 = h1 && is_leaf(ch.y);
+      const uint32_t n0 = l0 ? leaf_count(ch.x) : 0u;
+      const uint32_t n_leaf = n0 + (l1 ? leaf_count(ch.y) : 0u);
+      const uint32_t f0 = leaf_first(ch.x), f1 = leaf_first(ch.y) - n0;
+      for (uint32_t t = 0; t < n_leaf; ++t) {
+        DevPrim pr;
+        load_prim(prims, t < n0 ? f0 + t : f1 + t, &pr);
+        if (COUNT) {
+          if (fbits(pr.b[3]) & kTagCube) tc.cube++;
+          else tc.sphere++;
+        }
+        test_prim(r, pr, &bt, &bid);
       }
-      if (h1 && is_leaf(ch.y)) {
-        test_leaf<COUNT>(r, prims, ch.y, &bt, &bid, tc);
-        h1 = false;
-      }
+      h0 = h0 && !l0;
+      h1 = h1 && !l1;
       if (h0 && h1) {
         const bool first0 = t0 <= t1;
         stk[sp * BLOCK] = (StackT)(first0 ? ch.y : ch.x);
@@ -493,7 +522,7 @@ __global__ __launch_bounds__(BLOCK) void k_shade(IterArgs a, int p) {
       l_slot = T.l_offset + pidx;
       const uint32_t id = a.hit_id[k];
       if (id != kNoHit) {
-        const DevObject ob = J.objects[id];
+        const DevObject ob = load_object(J.objects, id);
         const HitRec h = closest_hit(o, d, a.hit_t[k], ob);
         const uint32_t npix = J.width * J.height;
         BounceDraws dr;
