@@ -218,6 +218,7 @@ bool option_value(const rd_renderer *r, const std::string &k, int64_t *v) {
   else if (k == "grid") *v = o.grid;
   else if (k == "traversal") *v = o.brute_force;
   else if (k == "refill") *v = o.refill;
+  else if (k == "lds_bvh") *v = o.lds_bvh;
   else if (k == "count") *v = o.count;
   else if (k == "kernel_timing") *v = o.kernel_timing;
   else if (k == "l_capacity") *v = (int64_t)o.l_capacity;
@@ -402,6 +403,9 @@ rd_status rd_set_option(rd_renderer *r, const char *key, int64_t v) {
   } else if (k == "refill") {
     if (v < 0 || v > 64) return r->fail(RD_ERR_INVALID_ARGUMENT, "refill must be in 0..64");
     o.refill = (int)v;
+  } else if (k == "lds_bvh") {
+    if (v < 0 || v > (1 << 17)) return r->fail(RD_ERR_INVALID_ARGUMENT, "lds_bvh must be in 0..131072");
+    o.lds_bvh = (uint32_t)v;
   } else if (k == "count") {
     o.count = v != 0;
   } else if (k == "kernel_timing") {

@@ -188,19 +188,23 @@ IterArgs Engine::iter_args(const DevJob *d_jobs) const {
   a.counters = counters;
   a.brute_force = opt.brute_force;
   a.refill = opt.refill;
+  a.lds_bvh_bytes = 0;
+  a.stack_rows = 0;
   a.single_job = 0;
   a.track_done = 1;
   a.wide_stack = 0;
   return a;
 }
 
-LaunchShape Engine::shape(int which) const {
+LaunchShape Engine::shape(int which, const IterArgs *a) const {
   LaunchShape s;
   s.block = opt.block;
   if (opt.grid > 0) {
     s.grid = opt.grid;
   } else {
-    const int per_cu = std::max(1, occupancy_blocks(which, s.block, which == 1 && opt.refill > 0));
+    const bool lanes = which == 1 && opt.refill > 0;
+    const uint32_t dyn = (which == 1 && a) ? extend_dynamic_lds(*a, s.block) : 0;
+    const int per_cu = std::max(1, occupancy_blocks(which, s.block, lanes, dyn ? a->lds_bvh_bytes : 0, dyn));
     s.grid = num_cus * per_cu;
   }
   return s;
@@ -210,12 +214,12 @@ bool Engine::timed(int which, std::string *err,
                    hipError_t (*fn)(const IterArgs &, int, const LaunchShape &, bool, hipStream_t),
                    const IterArgs &a, int parity) {
   const bool count = opt.count != 0;
-  if (!opt.kernel_timing || count) return check(fn(a, parity, shape(which), count, stream), "kernel launch", err);
+  if (!opt.kernel_timing || count) return check(fn(a, parity, shape(which, &a), count, stream), "kernel launch", err);
   hipEvent_t e0, e1;
   if (!check(hipEventCreate(&e0), "hipEventCreate", err)) return false;
   if (!check(hipEventCreate(&e1), "hipEventCreate", err)) return false;
   if (!check(hipEventRecord(e0, stream), "hipEventRecord", err)) return false;
-  if (!check(fn(a, parity, shape(which), false, stream), "kernel launch", err)) return false;
+  if (!check(fn(a, parity, shape(which, &a), false, stream), "kernel launch", err)) return false;
   if (!check(hipEventRecord(e1, stream), "hipEventRecord", err)) return false;
   if (!check(hipEventSynchronize(e1), "hipEventSynchronize", err)) return false;
   float ms = 0.f;
@@ -269,6 +273,15 @@ bool Engine::run_tickets(const DevJob *d_jobs, std::vector<DevTicket> tickets,
     a.track_done = 0;  // completion is known from the drained queue
     for (const DevJob &j : host_jobs)
       if (j.n_nodes > kNarrowStackNodes) a.wide_stack = 1;
+    if (a.single_job && !a.wide_stack && !opt.brute_force && opt.refill > 0) {
+      // a small scene is traversed from LDS, with a stack as deep as the tree
+      const DevJob &j = host_jobs[0];
+      const uint64_t bytes = (uint64_t)sizeof(DevNode) * j.n_nodes + (uint64_t)sizeof(DevPrim) * j.n_prims;
+      if (j.n_nodes > 0 && bytes <= opt.lds_bvh && j.depth + 1 <= (uint32_t)kStackDepth) {
+        a.lds_bvh_bytes = (uint32_t)bytes;
+        a.stack_rows = j.depth + 1;
+      }
+    }
     int parity = 0;
     const uint64_t cap_iters = (w / q_alloc + 2) * (uint64_t)(max_bounces + 1) + 64;
     uint64_t it = 0;
@@ -349,6 +362,7 @@ void fill_job(const rd_scene &s, const DeviceScene &ds, float *acc, uint32_t max
     j->solid[k] = s.world.solid_color[k];
   }
   j->r0_tol = ds.r0;
+  j->depth = ds.depth;
 }
 
 }  // namespace rdh

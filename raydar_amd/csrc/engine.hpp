@@ -37,6 +37,8 @@ struct EngineOptions {
   int grid = 0;                   // 0: occupancy-derived
   int brute_force = 0;
   int refill = 48;                // extend: persistent lanes refill at >= 48 idle (0: off)
+  uint32_t lds_bvh = 14336;       // single scenes with at most this many BVH bytes
+                                  // are traversed from LDS (0: never)
   int leaf = 1;                   // BVH leaf size; 1 measured fastest for one scene
   int count = 0;
   int kernel_timing = 0;
@@ -89,7 +91,7 @@ class Engine {
   unsigned long long *d_ticket_done = nullptr;
 
   IterArgs iter_args(const DevJob *d_jobs) const;
-  LaunchShape shape(int which) const;
+  LaunchShape shape(int which, const IterArgs *a = nullptr) const;
   bool launch_iteration(const IterArgs &a, int parity, std::string *err);
   bool ensure_tickets(uint32_t n, std::string *err);
 

@@ -338,12 +338,30 @@ void k_extend(IterArgs a, int p) {
 // as `refill` or more lanes of the wave are idle.  Each step of the loop
 // visits one BVH node per active lane.  Indices only grow per lane, so
 // slot_of's running segment stays valid.
-template <int BLOCK, bool COUNT, bool SINGLE, class StackT>
+//
+// LDS = true (single job, small scene): the workgroup first copies the BVH
+// nodes and primitives into LDS and traverses them there; the stack then
+// has only as many rows as the tree has levels, both in dynamic LDS.
+template <int BLOCK, bool COUNT, bool SINGLE, class StackT, bool LDS>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(RDH_EXTEND_WAVES, 8)))
 void k_extend_lanes(IterArgs a, int p) {
-  __shared__ StackT stack[kStackDepth * BLOCK];
+  __shared__ StackT static_stack[LDS ? 1 : kStackDepth * BLOCK];
+  extern __shared__ __align__(16) unsigned char dynamic_lds[];
   __shared__ SegPlan pl;
-  make_plan(a, p, pl);
+  StackT *stack = static_stack;
+  VecF4 *lds_nodes = nullptr, *lds_prims = nullptr;
+  if (LDS) {
+    const DevJob &J = a.jobs[0];
+    const uint32_t stack_bytes = (a.stack_rows * BLOCK * (uint32_t)sizeof(StackT) + 15u) & ~15u;
+    stack = reinterpret_cast<StackT *>(dynamic_lds);
+    lds_nodes = reinterpret_cast<VecF4 *>(dynamic_lds + stack_bytes);
+    lds_prims = lds_nodes + 4 * J.n_nodes;
+    GlobalF4 *gn = (GlobalF4 *)J.nodes;
+    GlobalF4 *gp = (GlobalF4 *)J.prims;
+    for (uint32_t i = threadIdx.x; i < 4 * J.n_nodes; i += BLOCK) lds_nodes[i] = gn[i];
+    for (uint32_t i = threadIdx.x; i < 2 * J.n_prims; i += BLOCK) lds_prims[i] = gp[i];
+  }
+  make_plan(a, p, pl);  // its barriers also publish the LDS copy
   // queue[1-p] is empty until shade fills it; nothing else reads it now
   if (blockIdx.x == 0 && threadIdx.x < kSegments) a.ctl->count[1 - p][threadIdx.x * kSegStride] = 0;
   if (blockIdx.x == 0 && threadIdx.x == 0) a.ctl->cursor[1 - p] = pl.cursor + pl.n_new;
@@ -424,9 +442,17 @@ void k_extend_lanes(IterArgs a, int p) {
       // one node: both child boxes, then the primitives of the hit leaf
       // children in one loop (lanes with a leaf on either side run it
       // together; the closest hit does not depend on the test order)
-      GlobalF4 *np = (GlobalF4 *)(nodes + node);
-      const VecF4 bx = np[0], by = np[1], bz = np[2];
-      const VecI4 ch = ((GlobalI4 *)np)[3];
+      VecF4 bx, by, bz;
+      VecI4 ch;
+      if (LDS) {
+        const VecF4 *np = lds_nodes + 4 * node;
+        bx = np[0]; by = np[1]; bz = np[2];
+        ch = __builtin_bit_cast(VecI4, np[3]);
+      } else {
+        GlobalF4 *np = (GlobalF4 *)(nodes + node);
+        bx = np[0]; by = np[1]; bz = np[2];
+        ch = ((GlobalI4 *)np)[3];
+      }
       const float tcull = cull_limit(bt, bid);
       float t0, t1;
       bool h0 = slab_hit(s, bx.x, bx.z, by.x, by.z, bz.x, bz.z, tcull, &t0);
@@ -438,7 +464,14 @@ void k_extend_lanes(IterArgs a, int p) {
       const uint32_t f0 = leaf_first(ch.x), f1 = leaf_first(ch.y) - n0;
       for (uint32_t t = 0; t < n_leaf; ++t) {
         DevPrim pr;
-        load_prim(prims, t < n0 ? f0 + t : f1 + t, &pr);
+        const uint32_t j = t < n0 ? f0 + t : f1 + t;
+        if (LDS) {
+          const VecF4 pa = lds_prims[2 * j], pb = lds_prims[2 * j + 1];
+          pr.a[0] = pa.x; pr.a[1] = pa.y; pr.a[2] = pa.z; pr.a[3] = pa.w;
+          pr.b[0] = pb.x; pr.b[1] = pb.y; pr.b[2] = pb.z; pr.b[3] = pb.w;
+        } else {
+          load_prim(prims, j, &pr);
+        }
         if (COUNT) {
           if (fbits(pr.b[3]) & kTagCube) tc.cube++;
           else tc.sphere++;
@@ -763,12 +796,14 @@ uint32_t blocks_for(uint64_t n, uint32_t block) { return (uint32_t)((n + block -
 // Kernel instance for a launch: block size, counting build, single-job
 // specialisation and stack width are template parameters.
 template <bool COUNT, bool SINGLE>
-static const void *extend_fn(int block, bool wide, bool lanes) {
-#define RDH_EXT(B)                                                               \
-  if (block == B) {                                                              \
-    if (lanes)                                                                   \
-      return wide ? (const void *)k_extend_lanes<B, COUNT, SINGLE, int32_t>      \
-                  : (const void *)k_extend_lanes<B, COUNT, SINGLE, uint16_t>;    \
+static const void *extend_fn(int block, bool wide, bool lanes, bool lds) {
+#define RDH_EXT(B)                                                                      \
+  if (block == B) {                                                                     \
+    if (lanes && lds && SINGLE && !wide)                                                \
+      return (const void *)k_extend_lanes<B, COUNT, true, uint16_t, true>;              \
+    if (lanes)                                                                          \
+      return wide ? (const void *)k_extend_lanes<B, COUNT, SINGLE, int32_t, false>      \
+                  : (const void *)k_extend_lanes<B, COUNT, SINGLE, uint16_t, false>;    \
     return wide ? (const void *)k_extend<B, COUNT, SINGLE, int32_t>              \
                 : (const void *)k_extend<B, COUNT, SINGLE, uint16_t>;            \
   }
@@ -786,39 +821,48 @@ static const void *shade_fn(int block) {
   return nullptr;
 }
 static const void *kernel_fn(int which, int block, bool count, bool single, bool wide,
-                             bool lanes = false) {
+                             bool lanes = false, bool lds = false) {
   if (which == 0) {
     if (count) return single ? (const void *)k_generate<true, true> : (const void *)k_generate<true, false>;
     return single ? (const void *)k_generate<false, true> : (const void *)k_generate<false, false>;
   }
   if (which == 1) {
     if (count)
-      return single ? extend_fn<true, true>(block, wide, lanes) : extend_fn<true, false>(block, wide, lanes);
-    return single ? extend_fn<false, true>(block, wide, lanes) : extend_fn<false, false>(block, wide, lanes);
+      return single ? extend_fn<true, true>(block, wide, lanes, lds)
+                    : extend_fn<true, false>(block, wide, lanes, lds);
+    return single ? extend_fn<false, true>(block, wide, lanes, lds)
+                  : extend_fn<false, false>(block, wide, lanes, lds);
   }
   if (count) return single ? shade_fn<true, true>(block) : shade_fn<true, false>(block);
   return single ? shade_fn<false, true>(block) : shade_fn<false, false>(block);
 }
 
-int occupancy_blocks(int which, int block, bool lanes) {
-  const void *fn = kernel_fn(which, block, false, true, false, lanes);
+int occupancy_blocks(int which, int block, bool lanes, uint32_t lds_bvh_bytes,
+                     uint32_t dynamic_lds) {
+  const void *fn = kernel_fn(which, block, false, true, false, lanes, lds_bvh_bytes > 0);
   int n = 0;
-  if (!fn || hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, block, 0) != hipSuccess) {
+  if (!fn || hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, block, dynamic_lds) != hipSuccess) {
     (void)hipGetLastError();
     return 1;
   }
   return n;
 }
 
+uint32_t extend_dynamic_lds(const IterArgs &a, int block) {
+  if (a.refill <= 0 || a.lds_bvh_bytes == 0 || !a.single_job || a.wide_stack) return 0;
+  return ((a.stack_rows * (uint32_t)block * 2u + 15u) & ~15u) + a.lds_bvh_bytes;
+}
+
 static hipError_t launch_stage(int which, const IterArgs &a, int parity, const LaunchShape &s,
                                bool count, hipStream_t st) {
+  const uint32_t dyn = which == 1 ? extend_dynamic_lds(a, s.block) : 0;
   const void *fn = kernel_fn(which, s.block, count, a.single_job != 0, a.wide_stack != 0,
-                             a.refill > 0);
+                             a.refill > 0, dyn > 0);
   if (!fn) return hipErrorInvalidValue;
   IterArgs args = a;
   int par = parity;
   void *kargs[] = {&args, &par};
-  return hipLaunchKernel(fn, dim3(s.grid), dim3(s.block), kargs, 0, st);
+  return hipLaunchKernel(fn, dim3(s.grid), dim3(s.block), kargs, dyn, st);
 }
 
 hipError_t launch_generate(const IterArgs &a, int parity, const LaunchShape &s, bool count,

@@ -40,7 +40,13 @@ struct IterArgs {
   int refill;                       // extend: 0 = one ray per lane per pass over the
                                     // queue; k > 0 = persistent lanes, a wave takes new
                                     // rays once k or more of its lanes are idle
+  uint32_t lds_bvh_bytes;           // > 0: single job whose nodes + primitives (this many
+                                    // bytes) each extend workgroup copies into LDS
+  uint32_t stack_rows;              // traversal stack entries per lane with lds_bvh_bytes
 };
+
+// Dynamic LDS of the extend launch for these arguments (0: static only).
+uint32_t extend_dynamic_lds(const IterArgs &a, int block);
 
 struct LaunchShape {
   int block = 256;   // 64, 128 or 256
@@ -58,7 +64,8 @@ hipError_t launch_accumulate(const DevJob *job, const DevTicket &t, const float4
 hipError_t launch_tonemap(const DevJob *job, uint32_t n_pixels, float spp, uchar4 *out,
                           DevCounters *counters, bool count, int grid, hipStream_t st);
 // Workgroups of kernel `which` (0 generate, 1 extend, 2 shade) that fit on one CU.
-int occupancy_blocks(int which, int block, bool lanes = false);
+int occupancy_blocks(int which, int block, bool lanes = false, uint32_t lds_bvh_bytes = 0,
+                     uint32_t dynamic_lds = 0);
 
 hipError_t launch_set_ctl(QueueCtl *ctl, unsigned long long total, uint32_t n_tickets,
                           hipStream_t st);

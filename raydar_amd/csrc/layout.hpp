@@ -82,7 +82,8 @@ struct DevJob {
   float position[4];
   float top[4], bottom[4], solid[4];
   float r0_tol;             // |o|_1 above which a ray needs the dynamic box margin
-  float pad_[3];
+  uint32_t depth;           // BVH levels (a traversal stack needs fewer entries)
+  float pad_[2];
 };
 
 struct DevTicket {

@@ -39,7 +39,8 @@ def _gpu_acc(scene, spp, bounces, seed, **opts):
 @pytest.mark.parametrize("opts", [dict(), dict(leaf=1, block=64, queue=4096),
                                   dict(leaf=8, block=128, queue=65536), dict(traversal=1),
                                   dict(refill=0), dict(refill=1, grid=7), dict(refill=64, leaf=2),
-                                  dict(refill=0, block=64, queue=4096)])
+                                  dict(refill=0, block=64, queue=4096), dict(lds_bvh=0),
+                                  dict(lds_bvh=0, refill=32, block=128)])
 def test_plain_small_matches_oracle(opts):
     sc = scenegen.plain_small(0)
     img, acc = _gpu_acc(sc, 16, 12, 5, **opts)
@@ -49,6 +50,16 @@ def test_plain_small_matches_oracle(opts):
     print(f"opts={opts}: accumulator bit-exact fraction {exact:.6f}")
     assert exact == 1.0
     np.testing.assert_array_equal(img.reshape(-1), oracle.tonemap(ref, 16))
+
+
+@pytest.mark.parametrize("lds_bvh", [0, 14336, 1 << 17])
+def test_lds_resident_bvh_matches_oracle(lds_bvh):
+    # 97 objects (9.3 KB of nodes + primitives): traversed from LDS unless disabled
+    sc = scenegen.plain_1080p(1, 96, 54)
+    img, acc = _gpu_acc(sc, 3, 12, 21, lds_bvh=lds_bvh)
+    ref = oracle.render(sc, 3, 12, 21)
+    np.testing.assert_array_equal(acc.view(np.uint32), ref.view(np.uint32))
+    np.testing.assert_array_equal(img.reshape(-1), oracle.tonemap(ref, 3))
 
 
 def test_default_rscn_matches_oracle():
