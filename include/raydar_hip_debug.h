@@ -31,7 +31,9 @@ rd_status rd_debug_trace(const rd_scene *scene, int leaf_size, int brute, const 
 
 /* Scatter of n hits on the GPU.  in: n x 16 floats = ray origin(3),
  * direction(3), t, object id (uint32 bits), attenuation(3), light(3), 2 unused;
- * bits: n x 12 draw words (slot order of cpu.rs:262-314).  out: n x 16 floats
+ * bits: n x 8 draw words, one bounce's two Philox blocks in slot order
+ * (0 transmission test, 1 Fresnel or metallic test, 2 roughness test,
+ * 4-6 the unit-sphere offset of the direction taken).  out: n x 16 floats
  * = new origin(3), new direction(3), attenuation(3), light(3), transmission
  * flag, normal(3). */
 rd_status rd_debug_scatter(const rd_scene *scene, const float *in, const uint32_t *bits,

@@ -61,8 +61,11 @@ def philox(ctr, key):
     return out
 
 
+DRAW_WORDS = 8  # random words per bounce: two Philox-4x32 blocks
+
+
 def bounce_bits(seed, pixel, sample, bounce):
-    out = np.zeros(12, np.uint32)
+    out = np.zeros(DRAW_WORDS, np.uint32)
     lib().or_bounce_bits(seed, pixel, sample, bounce, out)
     return out
 
@@ -88,9 +91,9 @@ def trace(scene, rays):
 
 
 def scatter(scene, packed, bits):
-    """packed: n x 16 floats (see oracle.cpp or_scatter_f32); bits: n x 12."""
+    """packed: n x 16 floats (see oracle.cpp or_scatter_f32); bits: n x 8."""
     packed = _c(packed, np.float32).reshape(-1, 16)
-    bits = _c(bits, np.uint32).reshape(-1, 12)
+    bits = _c(bits, np.uint32).reshape(-1, DRAW_WORDS)
     out = np.zeros_like(packed)
     s = scene.as_rd()
     lib().or_scatter_f32(C.byref(s), packed, bits, len(packed), out)

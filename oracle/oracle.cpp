@@ -22,7 +22,11 @@
 // reproducible).  Here every draw is an explicit uniform from a counter-based
 // Philox-4x32-10 stream (Salmon et al., SC'11) keyed by (seed) and counted by
 // (pixel, sample, bounce, slot), so the HIP kernels can consume identical
-// numbers.  Slot order per bounce follows cpu.rs:268,277,280,296,301,306,309.
+// numbers.  Per bounce: slot 0 transmission test (cpu.rs:279), 1 Fresnel
+// test (cpu.rs:295) or metallic test (cpu.rs:305), 2 roughness test
+// (cpu.rs:308), 4-6 the random_in_unit_sphere of the direction taken
+// (diffuse cpu.rs:266-267, specular cpu.rs:276 or refracted cpu.rs:300:
+// exclusive branches, so they share one triple).
 //
 // Build: -O2 -ffp-contract=off, no fast-math (Rust never contracts a*b+c).
 #include <cmath>
@@ -60,12 +64,13 @@ void philox4x32_10(const uint32_t in[4], const uint32_t key[2], uint32_t out[4])
   out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
 }
 
-// The 12 draw slots of one bounce: three Philox blocks (slot s is lane s%4 of
+// The 8 draw slots of one bounce: two Philox blocks (slot s is lane s%4 of
 // block s/4).  Counter = (pixel, sample, bounce*4 + block, 0); key = seed.
+constexpr int kDrawWords = 8;
 void bounce_bits(uint64_t seed, uint32_t pixel, uint32_t sample, uint32_t bounce,
-                 uint32_t bits[12]) {
+                 uint32_t bits[kDrawWords]) {
   const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
-  for (uint32_t b = 0; b < 3; ++b) {
+  for (uint32_t b = 0; b < 2; ++b) {
     const uint32_t ctr[4] = {pixel, sample, bounce * 4u + b, 0u};
     philox4x32_10(ctr, key, bits + 4 * b);
   }
@@ -207,19 +212,19 @@ template <class T> inline V3<T> world_sample(const rd_world &w, V3<T> d) {
 // Result of one scatter (cpu.rs:262-333).
 template <class T> struct Scatter { Ray<T> next; V3<T> atten, light; bool transmission; };
 
-// cpu.rs:262-333 with explicit slot uniforms (bits[0..11]).
+// cpu.rs:262-333 with explicit slot uniforms (bits[0..7]).
 template <class T>
 inline Scatter<T> scatter(const Ray<T> &ray, const Hit<T> &h, const rd_object &o,
                           const uint32_t *bits, V3<T> atten, V3<T> light) {
   T roughness = (T)o.roughness * (T)o.roughness;
   T metallic = (T)o.metallic;
   T transmission = (T)o.transmission;
-  V3<T> diffuse = h.n + random_in_unit_sphere<T>(bits + 0);
+  V3<T> diffuse = h.n + random_in_unit_sphere<T>(bits + 4);
   if (dot(diffuse, h.n) < T(0)) diffuse = -diffuse;
   V3<T> perfect = reflect(ray.d, h.n);
-  V3<T> offset = random_in_unit_sphere<T>(bits + 3) * roughness;
+  V3<T> offset = random_in_unit_sphere<T>(bits + 4) * roughness;
   V3<T> specular = normalize(perfect + offset);
-  bool trans = (T)unit_f32(bits[6]) < transmission;
+  bool trans = (T)unit_f32(bits[0]) < transmission;
   V3<T> dir;
   if (trans) {
     T ior = (T)o.ior;
@@ -232,16 +237,16 @@ inline Scatter<T> scatter(const Ray<T> &ray, const Hit<T> &h, const rd_object &o
     T y2 = y * y;
     T y5 = y * (y2 * y2);
     T refl = r0 + (T(1) - r0) * y5;
-    if (refl < (T)unit_f32(bits[7]) && can_refract(rd, h.n, ior)) {
+    if (refl < (T)unit_f32(bits[1]) && can_refract(rd, h.n, ior)) {
       V3<T> refr = refract(rd, h.n, ior);
-      V3<T> off2 = random_in_unit_sphere<T>(bits + 8) * roughness;
+      V3<T> off2 = random_in_unit_sphere<T>(bits + 4) * roughness;
       dir = normalize(refr + off2);
     } else {
       dir = specular;
     }
-  } else if ((T)unit_f32(bits[7]) < metallic) {
+  } else if ((T)unit_f32(bits[1]) < metallic) {
     dir = specular;
-  } else if ((T)unit_f32(bits[8]) < roughness) {
+  } else if ((T)unit_f32(bits[2]) < roughness) {
     dir = diffuse;
   } else {
     dir = specular;
@@ -294,7 +299,7 @@ inline V3<T> per_pixel(const rd_scene &s, uint32_t x, uint32_t y, uint32_t sampl
     if (id >= 0) {
       const rd_object &o = s.objects[id];
       Hit<T> h = closest_hit(ray, t, o);
-      uint32_t bits[12];
+      uint32_t bits[kDrawWords];
       bounce_bits(seed, pixel, sample, b, bits);
       Scatter<T> sc = scatter(ray, h, o, bits, atten, light);
       ray = sc.next;
@@ -338,7 +343,7 @@ void or_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out
 }
 
 void or_bounce_bits(uint64_t seed, uint32_t pixel, uint32_t sample, uint32_t bounce,
-                    uint32_t bits[12]) {
+                    uint32_t bits[kDrawWords]) {
   bounce_bits(seed, pixel, sample, bounce, bits);
 }
 
@@ -375,7 +380,7 @@ void or_trace_f32(const rd_scene *s, const float *rays, uint32_t n, int32_t *ids
 }
 
 // Scatter for n hits.  in: n x 16 floats = ray o(3) d(3), t, object id (as
-// float bits), atten(3), light(3), pad(2); bits: n x 12.  out: n x 16 floats =
+// float bits), atten(3), light(3), pad(2); bits: n x 8.  out: n x 16 floats =
 // next o(3) d(3), atten(3), light(3), transmission flag, hit p... (see tests).
 void or_scatter_f32(const rd_scene *s, const float *in, const uint32_t *bits, uint32_t n,
                     float *out) {
@@ -387,7 +392,7 @@ void or_scatter_f32(const rd_scene *s, const float *in, const uint32_t *bits, ui
     std::memcpy(&id, &q[7], 4);
     const rd_object &o = s->objects[id];
     Hit<float> h = closest_hit(r, t, o);
-    Scatter<float> sc = scatter(r, h, o, bits + 12 * i, mk(q[8], q[9], q[10]),
+    Scatter<float> sc = scatter(r, h, o, bits + kDrawWords * i, mk(q[8], q[9], q[10]),
                                 mk(q[11], q[12], q[13]));
     float *w = out + 16 * i;
     w[0] = sc.next.o.x; w[1] = sc.next.o.y; w[2] = sc.next.o.z;

@@ -95,6 +95,10 @@ def check(status, handle=None):
         raise RaydarError(status, last_error(handle))
 
 
+# Random words per bounce in the scatter hooks (two Philox-4x32 blocks).
+DRAW_WORDS = 8
+
+
 def _ptr(a):
     return a.ctypes.data_as(C.c_void_p)
 
@@ -251,7 +255,7 @@ def debug_trace(scene, rays, leaf=4, brute=False):
 
 def debug_scatter(scene, packed, bits):
     packed = np.ascontiguousarray(packed, np.float32).reshape(-1, 16)
-    bits = np.ascontiguousarray(bits, np.uint32).reshape(-1, 12)
+    bits = np.ascontiguousarray(bits, np.uint32).reshape(-1, DRAW_WORDS)
     out = np.zeros_like(packed)
     s = scene.as_rd()
     check(lib().rd_debug_scatter(C.byref(s), _ptr(packed), _ptr(bits), len(packed), _ptr(out)))
@@ -310,7 +314,7 @@ def host_trace(scene, rays, leaf=4):
 def host_scatter(scene, packed, bits):
     """The kernels' scatter compiled for the host (same layout as debug_scatter)."""
     packed = np.ascontiguousarray(packed, np.float32).reshape(-1, 16)
-    bits = np.ascontiguousarray(bits, np.uint32).reshape(-1, 12)
+    bits = np.ascontiguousarray(bits, np.uint32).reshape(-1, DRAW_WORDS)
     out = np.zeros_like(packed)
     s = scene.as_rd()
     check(lib().rd_debug_host_scatter(C.byref(s), _ptr(packed), _ptr(bits), len(packed), _ptr(out)))

@@ -219,6 +219,7 @@ bool option_value(const rd_renderer *r, const std::string &k, int64_t *v) {
   else if (k == "traversal") *v = o.brute_force;
   else if (k == "refill") *v = o.refill;
   else if (k == "lds_bvh") *v = o.lds_bvh;
+  else if (k == "shade_blocks") *v = o.shade_blocks;
   else if (k == "count") *v = o.count;
   else if (k == "kernel_timing") *v = o.kernel_timing;
   else if (k == "l_capacity") *v = (int64_t)o.l_capacity;
@@ -403,6 +404,9 @@ rd_status rd_set_option(rd_renderer *r, const char *key, int64_t v) {
   } else if (k == "refill") {
     if (v < 0 || v > 64) return r->fail(RD_ERR_INVALID_ARGUMENT, "refill must be in 0..64");
     o.refill = (int)v;
+  } else if (k == "shade_blocks") {
+    if (v < 0 || v > 64) return r->fail(RD_ERR_INVALID_ARGUMENT, "shade_blocks must be in 0..64");
+    o.shade_blocks = (int)v;
   } else if (k == "lds_bvh") {
     if (v < 0 || v > (1 << 17)) return r->fail(RD_ERR_INVALID_ARGUMENT, "lds_bvh must be in 0..131072");
     o.lds_bvh = (uint32_t)v;
@@ -570,7 +574,7 @@ rd_status rd_debug_scatter(const rd_scene *scene, const float *in, const uint32_
   rd_status st = debug_setup(scene, 4, &c);
   if (st != RD_OK) return st;
   const float *d_in = c.dev(in, 16ull * n);
-  const uint32_t *d_bits = c.dev(bits, 12ull * n);
+  const uint32_t *d_bits = c.dev(bits, (uint64_t)rdh::kDrawWords * n);
   float *d_out = c.dev<float>(nullptr, 16ull * n);
   if (!d_in || !d_bits || !d_out) return RD_ERR_OUT_OF_MEMORY;
   if ((st = finish(rdh::launch_debug_scatter(c.objects, d_in, d_bits, n, d_out, nullptr))) != RD_OK)
@@ -693,7 +697,7 @@ rd_status rd_debug_host_scatter(const rd_scene *scene, const float *in, const ui
     const rdh::F3 o = rdh::f3(q[0], q[1], q[2]), d = rdh::f3(q[3], q[4], q[5]);
     const rdh::DevObject &ob = b.objects[id];
     const rdh::HitRec h = rdh::closest_hit(o, d, q[6], ob);
-    rdh::FixedDraws dr{bits + 12ull * i};
+    rdh::FixedDraws dr{bits + (uint64_t)rdh::kDrawWords * i};
     const rdh::ScatterOut so = rdh::scatter(d, h, ob, dr, rdh::f3(q[8], q[9], q[10]),
                                             rdh::f3(q[11], q[12], q[13]));
     float *w = out + 16ull * i;

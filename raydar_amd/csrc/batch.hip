@@ -214,7 +214,7 @@ class Streamer {
   uint64_t fixed_bytes_ = 0;
   hipGraphExec_t graph_ = nullptr;
   hipGraph_t graph_def_ = nullptr;
-  int graph_block_ = 0, graph_grid_ = 0, graph_brute_ = 0, graph_refill_ = -1;
+  int graph_block_ = 0, graph_grid_ = 0, graph_brute_ = 0, graph_refill_ = -1, graph_shade_ = -1;
 
   void teardown() {
     if (graph_) (void)hipGraphExecDestroy(graph_);
@@ -365,6 +365,7 @@ class Streamer {
     graph_grid_ = eng.opt.grid;
     graph_brute_ = eng.opt.brute_force;
     graph_refill_ = eng.opt.refill;
+    graph_shade_ = eng.opt.shade_blocks;
     graph_wide_ = wide_;
     graph_gen_ = eng.generation;
     return true;
@@ -384,7 +385,8 @@ class Streamer {
       return true;
     }
     if (!graph_ || graph_block_ != eng.opt.block || graph_grid_ != eng.opt.grid ||
-        graph_brute_ != eng.opt.brute_force || graph_refill_ != eng.opt.refill || graph_wide_ != wide_ ||
+        graph_brute_ != eng.opt.brute_force || graph_refill_ != eng.opt.refill || graph_shade_ != eng.opt.shade_blocks ||
+        graph_wide_ != wide_ ||
         graph_gen_ != eng.generation) {
       if (!capture()) return false;
     }
@@ -537,6 +539,9 @@ bool engine_option(rdh::Streamer &st, const std::string &k, int64_t v, std::stri
   } else if (k == "refill") {
     if (v < 0 || v > 64) { *err = "refill must be in 0..64"; return false; }
     o.refill = (int)v;
+  } else if (k == "shade_blocks") {
+    if (v < 0 || v > 64) { *err = "shade_blocks must be in 0..64"; return false; }
+    o.shade_blocks = (int)v;
   } else if (k == "count") {
     o.count = v != 0;
   } else if (k == "kernel_timing") {
@@ -555,6 +560,7 @@ bool engine_option_get(const rdh::Streamer &st, const std::string &k, int64_t *v
   else if (k == "leaf") *v = o.leaf;
   else if (k == "grid") *v = o.grid;
   else if (k == "refill") *v = o.refill;
+  else if (k == "shade_blocks") *v = o.shade_blocks;
   else if (k == "graph") *v = st.use_graph;
   else if (k == "iters_per_pump") *v = st.iters_per_pump;
   else if (k == "l_capacity") *v = (int64_t)o.l_capacity;

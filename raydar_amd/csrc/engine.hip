@@ -204,7 +204,8 @@ LaunchShape Engine::shape(int which, const IterArgs *a) const {
   } else {
     const bool lanes = which == 1 && opt.refill > 0;
     const uint32_t dyn = (which == 1 && a) ? extend_dynamic_lds(*a, s.block) : 0;
-    const int per_cu = std::max(1, occupancy_blocks(which, s.block, lanes, dyn ? a->lds_bvh_bytes : 0, dyn));
+    int per_cu = std::max(1, occupancy_blocks(which, s.block, lanes, dyn ? a->lds_bvh_bytes : 0, dyn));
+    if (which == 2 && opt.shade_blocks > 0) per_cu = std::min(per_cu, opt.shade_blocks);
     s.grid = num_cus * per_cu;
   }
   return s;

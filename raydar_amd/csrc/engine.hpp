@@ -39,6 +39,7 @@ struct EngineOptions {
   int refill = 48;                // extend: persistent lanes refill at >= 48 idle (0: off)
   uint32_t lds_bvh = 14336;       // single scenes with at most this many BVH bytes
                                   // are traversed from LDS (0: never)
+  int shade_blocks = 0;           // shade workgroups per CU (0: as many as fit)
   int leaf = 1;                   // BVH leaf size; 1 measured fastest for one scene
   int count = 0;
   int kernel_timing = 0;

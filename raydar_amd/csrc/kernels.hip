@@ -378,6 +378,9 @@ void k_extend_lanes(IterArgs a, int p) {
   const DevJob &J0 = a.jobs[0];
   const DevNode *nodes = J0.nodes;
   const DevPrim *prims = J0.prims;
+  // job-0 fields read once (a single-job launch needs no per-ray job loads)
+  const uint32_t j0_prims = J0.n_prims, j0_nodes = J0.n_nodes;
+  const float j0_r0 = J0.r0_tol;
   bool active = false;
   uint32_t k = 0, seg = 0, bid = kNoHit;
   int32_t node = 0;
@@ -402,17 +405,19 @@ void k_extend_lanes(IterArgs a, int p) {
         const F3 o = f3(q.ox[k], q.oy[k], q.oz[k]);
         const F3 d = f3(q.dx[k], q.dy[k], q.dz[k]);
         const DevJob &J = a.jobs[SINGLE ? 0u : a.tickets[q.tb[k] >> 8].job];
+        const uint32_t n_prims = SINGLE ? j0_prims : J.n_prims;
+        const uint32_t n_nodes = SINGLE ? j0_nodes : J.n_nodes;
         r = ray_pre(o, d);
         bt = 0.0f;
         bid = kNoHit;
         const bool finite = isfinite(r.ox) && isfinite(r.oy) && isfinite(r.oz) && isfinite(r.dx) &&
                             isfinite(r.dy) && isfinite(r.dz);
-        if (J.n_prims == 0 || a.brute_force || !finite || J.n_nodes == 0) {
+        if (n_prims == 0 || a.brute_force || !finite || n_nodes == 0) {
           // the reference's linear scan (cpu.rs:345-349), also the exact
           // answer for non-finite rays; done at once
-          for (uint32_t j = 0; j < J.n_prims; ++j) {
+          for (uint32_t j = 0; j < n_prims; ++j) {
             DevPrim pr;
-            load_prim(J.prims, j, &pr);
+            load_prim(SINGLE ? prims : J.prims, j, &pr);
             if (COUNT) {
               if (fbits(pr.b[3]) & kTagCube) tc.cube++;
               else tc.sphere++;
@@ -427,7 +432,7 @@ void k_extend_lanes(IterArgs a, int p) {
             nodes = J.nodes;
             prims = J.prims;
           }
-          s = slab_ray(r, ray_tolerance(r, J.r0_tol));
+          s = slab_ray(r, ray_tolerance(r, SINGLE ? j0_r0 : J.r0_tol));
           node = 0;
           sp = 0;
           active = true;
@@ -745,7 +750,7 @@ __global__ void k_debug_scatter(const DevObject *objects, const float *in, const
   const uint32_t id = fbits(q[7]);
   const DevObject ob = objects[id];
   const HitRec h = closest_hit(o, d, q[6], ob);
-  FixedDraws dr{bits + 12 * (uint64_t)i};
+  FixedDraws dr{bits + kDrawWords * (uint64_t)i};
   const ScatterOut so = scatter(d, h, ob, dr, f3(q[8], q[9], q[10]), f3(q[11], q[12], q[13]));
   float *w = out + 16 * (uint64_t)i;
   w[0] = so.o.x; w[1] = so.o.y; w[2] = so.o.z;

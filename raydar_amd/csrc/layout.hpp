@@ -29,6 +29,7 @@ constexpr int kMaxLeafSize = 8;
 constexpr int kStackDepth = 32;          // >= max BVH depth the builder emits
 constexpr uint32_t kNarrowStackNodes = 65535;  // 16-bit LDS stack entries suffice
 constexpr uint32_t kNoHit = 0x7fffffffu; // best-id sentinel: "no hit yet"
+constexpr int kDrawWords = 8;            // random words per bounce (two Philox blocks)
 
 // Leaf child code: bit 31 set, first prim in bits 4..30, count in bits 0..3.
 __host__ __device__ inline bool is_leaf(int32_t c) { return c < 0; }

@@ -73,16 +73,20 @@ RD_HD U4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint3
 
 // Draw slots of one bounce.  Slot s is word s%4 of Philox block s/4 with
 // counter (pixel, sample, bounce*4 + block, 0) and key (seed lo, seed hi).
-// Slots (cpu.rs:262-314): 0-2 diffuse random_in_unit_sphere, 3-5 specular
-// offset, 6 transmission test, 7 Fresnel test (glass) or metallic test,
-// 8-10 refraction offset (glass), 8 roughness test (non-metal, non-glass).
-// Blocks 0 and 1 are always consumed; block 2 is generated on demand.
-// Each block is generated the first time one of its slots is read: a draw's
-// value depends only on its counter, so skipping unread blocks changes no
-// result.
+//   block 0: 0 transmission test (cpu.rs:279), 1 Fresnel test (glass,
+//            cpu.rs:295) or metallic test (cpu.rs:305), 2 roughness test
+//            (cpu.rs:308), 3 unused
+//   block 1: 4-6 the random_in_unit_sphere of the direction the bounce
+//            takes: diffuse (cpu.rs:266-267), specular offset (cpu.rs:276)
+//            or refraction offset (cpu.rs:300); 7 unused
+// The reference draws both the diffuse and the specular offsets before it
+// picks one; the branches are exclusive, so one triple serves whichever is
+// taken and every hit costs two Philox blocks.  A block is generated the
+// first time one of its slots is read (a draw depends only on its counter,
+// so the order of generation changes no result).
 struct BounceDraws {
   uint32_t pixel, sample, bounce, k0, k1;
-  U4 b[3];
+  U4 b[2];
   uint32_t have;
   RD_HD void init(uint64_t seed, uint32_t px, uint32_t smp, uint32_t bnc) {
     pixel = px; sample = smp; bounce = bnc;
@@ -311,7 +315,7 @@ RD_HD ScatterOut scatter(F3 ray_d, const HitRec &h, const DevObject &ob, Draws &
   const float rough = ob.m0[3];  // roughness^2, precomputed with the same multiply
   const float metallic = ob.m1[3];
   const float transmission = ob.m2[0];
-  const bool trans = unit_f32(dr.slot(6)) < transmission;
+  const bool trans = unit_f32(dr.slot(0)) < transmission;
   enum { kSpecular, kDiffuse, kRefract } pick = kSpecular;
   F3 rd = f3(0.0f, 0.0f, 0.0f);
   float eta = 1.0f;
@@ -326,23 +330,23 @@ RD_HD ScatterOut scatter(F3 ray_d, const HitRec &h, const DevObject &ob, Draws &
     const float y2 = y * y;
     const float y5 = y * (y2 * y2);
     const float refl = r0 + (1.0f - r0) * y5;
-    if (refl < unit_f32(dr.slot(7)) && can_refract(rd, h.n, ior)) pick = kRefract;
+    if (refl < unit_f32(dr.slot(1)) && can_refract(rd, h.n, ior)) pick = kRefract;
     eta = ior;
-  } else if (unit_f32(dr.slot(7)) < metallic) {
+  } else if (unit_f32(dr.slot(1)) < metallic) {
     pick = kSpecular;
-  } else if (unit_f32(dr.slot(8)) < rough) {
+  } else if (unit_f32(dr.slot(2)) < rough) {
     pick = kDiffuse;
   }
   F3 dir;
   if (pick == kRefract) {
     const F3 refr = refract(rd, h.n, eta);
-    dir = normalize(add(refr, scale(unit_sphere_draw(dr, 8), rough)));
+    dir = normalize(add(refr, scale(unit_sphere_draw(dr, 4), rough)));
   } else if (pick == kDiffuse) {
-    dir = add(h.n, unit_sphere_draw(dr, 0));
+    dir = add(h.n, unit_sphere_draw(dr, 4));
     if (dot(dir, h.n) < 0.0f) dir = neg(dir);
   } else {
     const F3 perfect = reflect(ray_d, h.n);
-    dir = normalize(add(perfect, scale(unit_sphere_draw(dr, 3), rough)));
+    dir = normalize(add(perfect, scale(unit_sphere_draw(dr, 4), rough)));
   }
   ScatterOut out;
   const F3 off = trans ? dir : h.n;

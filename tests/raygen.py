@@ -104,5 +104,5 @@ def scatter_inputs(scene, n, seed):
     packed[:, 7] = ids[h].astype(np.uint32).view(np.float32)
     packed[:, 8:11] = rng.uniform(0, 1, (m, 3))
     packed[:, 11:14] = rng.uniform(0, 3, (m, 3))
-    bits = rng.integers(0, 2**32, (m, 12), dtype=np.uint64).astype(np.uint32)
+    bits = rng.integers(0, 2**32, (m, 8), dtype=np.uint64).astype(np.uint32)
     return packed, bits

@@ -43,7 +43,8 @@ def test_bounce_bits_layout():
     # slot s = word s%4 of block s/4 with counter (pixel, sample, bounce*4+block, 0)
     seed = 0x0123456789ABCDEF
     bits = oracle.bounce_bits(seed, 17, 5, 3)
-    for b in range(3):
+    assert len(bits) == 8
+    for b in range(2):
         ref = oracle.philox([17, 5, 3 * 4 + b, 0], [seed & 0xffffffff, seed >> 32])
         assert list(bits[4 * b:4 * b + 4]) == list(ref)
 
@@ -195,7 +196,7 @@ def _pack(rays, t, ids, atten=(1, 1, 1), light=(0, 0, 0)):
 
 
 def _bits(**slots):
-    b = np.zeros(12, np.uint32)
+    b = np.zeros(oracle.DRAW_WORDS, np.uint32)
     for k, v in slots.items():
         b[int(k[1:])] = v
     return b
@@ -215,7 +216,7 @@ def test_mirror_reflection_preserves_length_and_angle():
     rays = np.concatenate([o, d], 1).astype(np.float32)
     ids, ts, nrm, _ = oracle.trace(sc, rays)
     h = ids >= 0
-    bits = np.tile(_bits(s7=U(0.5)), (h.sum(), 1))  # metallic branch: u7 < 1
+    bits = np.tile(_bits(s1=U(0.5)), (h.sum(), 1))  # metallic branch: u1 < 1
     out = oracle.scatter(sc, _pack(rays[h], ts[h], ids[h]), bits)
     nd = out[:, 3:6]
     assert np.allclose(np.linalg.norm(nd, axis=1), 1, atol=1e-6)
@@ -237,7 +238,7 @@ def test_refraction_obeys_snell_and_schlick():
     rays[:, 5] = -1
     ids, ts, nrm, front = oracle.trace(sc, rays)
     assert np.all(ids == 0) and np.all(front == 1)
-    bits = np.tile(_bits(s6=U(0.5), s7=U(0.999)), (400, 1))  # refract unless R > 0.999
+    bits = np.tile(_bits(s0=U(0.5), s1=U(0.999)), (400, 1))  # refract unless R > 0.999
     out = oracle.scatter(sc, _pack(rays, ts, ids), bits)
     assert np.all(out[:, 12] == 1)
     d_in, d_out, n = rays[:, 3:].astype(np.float64), out[:, 3:6].astype(np.float64), nrm.astype(np.float64)
@@ -250,8 +251,8 @@ def test_refraction_obeys_snell_and_schlick():
     # Schlick at normal incidence: R = r0 = ((1/1.5 - 1)/(1/1.5 + 1))^2 = 0.04
     axial = np.array([[0, 0, 5, 0, 0, -1]], np.float32)
     i2, t2, _, _ = oracle.trace(sc, axial)
-    refl = oracle.scatter(sc, _pack(axial, t2, i2), _bits(s6=U(0.5), s7=U(0.0399))[None])
-    refr = oracle.scatter(sc, _pack(axial, t2, i2), _bits(s6=U(0.5), s7=U(0.0401))[None])
+    refl = oracle.scatter(sc, _pack(axial, t2, i2), _bits(s0=U(0.5), s1=U(0.0399))[None])
+    refr = oracle.scatter(sc, _pack(axial, t2, i2), _bits(s0=U(0.5), s1=U(0.0401))[None])
     assert refl[0, 5] > 0.99 and refr[0, 5] < -0.99
 
 
@@ -265,7 +266,7 @@ def test_total_internal_reflection():
     p = ray[0, :3] + ray[0, 3:] * ts[0]
     cos_t = -np.dot(ray[0, 3:], nrm[0])
     assert 1.5 * math.sqrt(1 - cos_t ** 2) > 1  # beyond the critical angle
-    out = oracle.scatter(sc, _pack(ray, ts, ids), _bits(s6=U(0.5), s7=U(0.9999))[None])
+    out = oracle.scatter(sc, _pack(ray, ts, ids), _bits(s0=U(0.5), s1=U(0.9999))[None])
     refl = ray[0, 3:] - 2 * np.dot(ray[0, 3:], nrm[0]) * nrm[0]
     assert np.allclose(out[0, 3:6], refl / np.linalg.norm(refl), atol=2e-6)
     assert np.allclose(out[0, :3], p + out[0, 3:6] * 1e-4, atol=1e-6)
@@ -280,10 +281,10 @@ def test_diffuse_direction_hemisphere_and_unit_offset():
     rays[:, 2] = 5
     rays[:, 5] = -1
     ids, ts, nrm, _ = oracle.trace(sc, rays)
-    bits = rng.integers(0, 2**32, (n, 12), dtype=np.uint64).astype(np.uint32)
-    bits[:, 6] = U(0.5)
-    bits[:, 7] = U(0.5)
-    bits[:, 8] = U(0.5)  # roughness^2 = 1 > u8: diffuse branch
+    bits = rng.integers(0, 2**32, (n, oracle.DRAW_WORDS), dtype=np.uint64).astype(np.uint32)
+    bits[:, 0] = U(0.5)
+    bits[:, 1] = U(0.5)
+    bits[:, 2] = U(0.5)  # roughness^2 = 1 > u2: diffuse branch
     out = oracle.scatter(sc, _pack(rays, ts, ids), bits)
     dd = out[:, 3:6]
     assert np.all(np.sum(dd * nrm, 1) >= 0)
