@@ -385,7 +385,7 @@ void k_extend_lanes(IterArgs a, int p) {
   uint32_t k = 0, seg = 0, bid = kNoHit;
   int32_t node = 0;
   int sp = 0;
-  float bt = 0.0f;
+  float bt = 0.0f, tcull = INFINITY;
   RayPre r;
   SlabRay s;
   TraceCounts tc;
@@ -435,6 +435,7 @@ void k_extend_lanes(IterArgs a, int p) {
           s = slab_ray(r, ray_tolerance(r, SINGLE ? j0_r0 : J.r0_tol));
           node = 0;
           sp = 0;
+          tcull = INFINITY;  // cull_limit of "no hit yet"
           active = true;
         }
       }
@@ -458,45 +459,48 @@ void k_extend_lanes(IterArgs a, int p) {
         bx = np[0]; by = np[1]; bz = np[2];
         ch = ((GlobalI4 *)np)[3];
       }
-      const float tcull = cull_limit(bt, bid);
       float t0, t1;
-      bool h0 = slab_hit(s, bx.x, bx.z, by.x, by.z, bz.x, bz.z, tcull, &t0);
-      bool h1 = slab_hit(s, bx.y, bx.w, by.y, by.w, bz.y, bz.w, tcull, &t1);
+      const bool h0 = slab_hit(s, bx.x, bx.z, by.x, by.z, bz.x, bz.z, tcull, &t0);
+      const bool h1 = slab_hit(s, bx.y, bx.w, by.y, by.w, bz.y, bz.w, tcull, &t1);
       if (COUNT) tc.box += 2;
       const bool l0 = h0 && is_leaf(ch.x), l1 = h1 && is_leaf(ch.y);
       const uint32_t n0 = l0 ? leaf_count(ch.x) : 0u;
       const uint32_t n_leaf = n0 + (l1 ? leaf_count(ch.y) : 0u);
-      const uint32_t f0 = leaf_first(ch.x), f1 = leaf_first(ch.y) - n0;
-      for (uint32_t t = 0; t < n_leaf; ++t) {
-        DevPrim pr;
-        const uint32_t j = t < n0 ? f0 + t : f1 + t;
-        if (LDS) {
-          const VecF4 pa = lds_prims[2 * j], pb = lds_prims[2 * j + 1];
-          pr.a[0] = pa.x; pr.a[1] = pa.y; pr.a[2] = pa.z; pr.a[3] = pa.w;
-          pr.b[0] = pb.x; pr.b[1] = pb.y; pr.b[2] = pb.z; pr.b[3] = pb.w;
-        } else {
-          load_prim(prims, j, &pr);
+      if (n_leaf) {
+        const uint32_t f0 = leaf_first(ch.x), f1 = leaf_first(ch.y) - n0;
+        for (uint32_t t = 0; t < n_leaf; ++t) {
+          DevPrim pr;
+          const uint32_t j = t < n0 ? f0 + t : f1 + t;
+          if (LDS) {
+            const VecF4 pa = lds_prims[2 * j], pb = lds_prims[2 * j + 1];
+            pr.a[0] = pa.x; pr.a[1] = pa.y; pr.a[2] = pa.z; pr.a[3] = pa.w;
+            pr.b[0] = pb.x; pr.b[1] = pb.y; pr.b[2] = pb.z; pr.b[3] = pb.w;
+          } else {
+            load_prim(prims, j, &pr);
+          }
+          if (COUNT) {
+            if (fbits(pr.b[3]) & kTagCube) tc.cube++;
+            else tc.sphere++;
+          }
+          test_prim(r, pr, &bt, &bid);
         }
-        if (COUNT) {
-          if (fbits(pr.b[3]) & kTagCube) tc.cube++;
-          else tc.sphere++;
-        }
-        test_prim(r, pr, &bt, &bid);
+        tcull = cull_limit(bt, bid);  // changes only when a primitive is tested
       }
-      h0 = h0 && !l0;
-      h1 = h1 && !l1;
-      if (h0 && h1) {
-        const bool first0 = t0 <= t1;
-        stk[sp * BLOCK] = (StackT)(first0 ? ch.y : ch.x);
-        ++sp;
-        node = first0 ? ch.x : ch.y;
-      } else if (h0) {
-        node = ch.x;
-      } else if (h1) {
-        node = ch.y;
+      // next node without branches: the far child is written above the
+      // stack top every step (kept only when both children are entered;
+      // slot sp < depth is in bounds) and the top entry is read every step
+      const bool i0 = h0 && !l0, i1 = h1 && !l1;
+      const bool near0 = i0 && (!i1 || t0 <= t1);
+      const int32_t near_c = near0 ? ch.x : ch.y;
+      stk[sp * BLOCK] = (StackT)(near0 ? ch.y : ch.x);
+      const int top = sp > 0 ? sp - 1 : 0;
+      const int32_t popped = (int32_t)stk[top * BLOCK];
+      if (i0 || i1) {
+        node = near_c;
+        sp += (i0 && i1) ? 1 : 0;
       } else if (sp > 0) {
-        --sp;
-        node = (int32_t)stk[sp * BLOCK];
+        node = popped;
+        sp = top;
       } else {
         a.hit_t[k] = bid == kNoHit ? 0.0f : bt;
         a.hit_id[k] = bid;
