@@ -191,9 +191,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=10.0)
     ap.add_argument("--no-batched", action="store_true")
-    ap.add_argument("--no-autotune", action="store_true")
-    ap.add_argument("--serve-seconds", type=float, default=30.0,
-                    help="length of the Poisson serving run (0 skips it)")
+    ap.add_argument("--autotune", action="store_true",
+                    help="autotune the batched renderer on a sample of the jobs first")
+    ap.add_argument("--serve-seconds", type=float, default=0.0,
+                    help="length of a Poisson serving run (default 0: skipped)")
     ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--option", action="append", default=[], help="key=value renderer option")
     ap.add_argument("--timed-only", action="store_true",
@@ -314,7 +315,7 @@ def main():
     r.close()
     if not args.no_batched:
         print("# batched vs plain...", file=sys.stderr, flush=True)
-        result["batched"] = batched_vs_plain(autotune=not args.no_autotune)
+        result["batched"] = batched_vs_plain(autotune=args.autotune)
     if args.serve_seconds > 0:
         print(f"# serving {args.serve_seconds} s...", file=sys.stderr, flush=True)
         result["serving"] = serving(args.serve_seconds, args.serve_rate)
