@@ -1,6 +1,9 @@
 """Build libraydar_hip.so (HIP kernels for gfx950 + host C++) in-tree.
 
-    python -m raydar_amd.build
+    python -m raydar_amd.build            # libraydar_hip.so
+    python -m raydar_amd.build --checked  # libraydar_hip_checked.so: every kernel
+                                          # index bounds-checked (RDH_CHECKED); load
+                                          # it with RAYDAR_HIP_LIB=<path>
 
 HIP sources are compiled with hipcc for gfx950 only, with
   -ffp-contract=off          no a*b+c fusion: the kernels must reproduce the
@@ -46,24 +49,29 @@ def _run(cmd):
     return out
 
 
-def build(verbose=False):
-    os.makedirs(BUILD, exist_ok=True)
+def build(verbose=False, checked=False):
+    out_dir = BUILD + ("_checked" if checked else "")
+    os.makedirs(out_dir, exist_ok=True)
     jobs = []
     for s in HIP_SOURCES:
         src = os.path.join(CSRC, s)
         if os.path.exists(src):
-            jobs.append([HIPCC, *HIP_FLAGS, "-c", src, "-o", os.path.join(BUILD, s + ".o")])
+            extra = ["-DRDH_CHECKED=1"] if checked else []
+            jobs.append([HIPCC, *HIP_FLAGS, *extra, "-c", src, "-o", os.path.join(out_dir, s + ".o")])
     for s in CXX_SOURCES:
         jobs.append(["g++", *CXX_FLAGS, "-c", os.path.join(CSRC, s), "-o",
-                     os.path.join(BUILD, s + ".o")])
+                     os.path.join(out_dir, s + ".o")])
     with ThreadPoolExecutor(max_workers=min(8, len(jobs))) as ex:
         for out in ex.map(_run, jobs):
             if verbose and out.strip():
                 print(out)
     objs = [j[-1] for j in jobs]
-    tmp = LIB + ".tmp"
+    lib = LIB.replace(".so", "_checked.so") if checked else LIB
+    tmp = lib + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
-    os.replace(tmp, LIB)
+    os.replace(tmp, lib)
+    if checked:
+        return lib
     cli_src = os.path.join(CSRC, "raydar_cli.cpp")
     if os.path.exists(cli_src):
         _run(["g++", *CXX_FLAGS, "-O2", "-o", CLI + ".tmp", cli_src, "-L" + HERE,
@@ -82,4 +90,4 @@ def clean():
 if __name__ == "__main__":
     if "--clean" in sys.argv:
         clean()
-    print(build(verbose="-v" in sys.argv))
+    print(build(verbose="-v" in sys.argv, checked="--checked" in sys.argv))

@@ -336,6 +336,7 @@ class Streamer {
   IterArgs args() const {
     IterArgs a = eng.iter_args(d_jobs_);
     a.single_job = 0;
+    a.job_cap = kJobSlots;
     a.track_done = 1;
     a.wide_stack = wide_ ? 1 : 0;
     return a;
@@ -412,6 +413,11 @@ class Streamer {
     if (!eng.check(hipStreamSynchronize(eng.stream), "iteration", &err)) return false;
     if (eng.ctl_host->overflow) {
       err = "path queue segment overflow (internal error)";
+      return false;
+    }
+    if (eng.ctl_host->violation) {
+      err = "device bounds check failed, mask " + std::to_string(eng.ctl_host->violation) +
+            " (internal error)";
       return false;
     }
     const uint32_t count = Engine::queued(*eng.ctl_host, parity_);

@@ -190,6 +190,10 @@ IterArgs Engine::iter_args(const DevJob *d_jobs) const {
   a.refill = opt.refill;
   a.lds_bvh_bytes = 0;
   a.stack_rows = 0;
+  a.slots = (uint32_t)kSegments * seg_cap;
+  a.ticket_cap = ticket_alloc;
+  a.job_cap = 1;
+  a.l_cap = l_alloc;
   a.single_job = 0;
   a.track_done = 1;
   a.wide_stack = 0;
@@ -271,6 +275,7 @@ bool Engine::run_tickets(const DevJob *d_jobs, std::vector<DevTicket> tickets,
     if (!check(launch_set_ctl(ctl, w, n, stream), "set ctl", err)) return false;
     IterArgs a = iter_args(d_jobs);
     a.single_job = host_jobs.size() == 1 ? 1 : 0;
+    a.job_cap = (uint32_t)host_jobs.size();
     a.track_done = 0;  // completion is known from the drained queue
     for (const DevJob &j : host_jobs)
       if (j.n_nodes > kNarrowStackNodes) a.wide_stack = 1;
@@ -299,6 +304,11 @@ bool Engine::run_tickets(const DevJob *d_jobs, std::vector<DevTicket> tickets,
       if (!check(hipStreamSynchronize(stream), "iteration", err)) return false;
       if (ctl_host->overflow) {
         *err = "path queue segment overflow (internal error)";
+        return false;
+      }
+      if (ctl_host->violation) {
+        *err = "device bounds check failed, mask " + std::to_string(ctl_host->violation) +
+               " (internal error)";
         return false;
       }
       if (queued(*ctl_host, parity) == 0 && ctl_host->cursor[parity] == w) break;

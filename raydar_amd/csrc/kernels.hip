@@ -13,6 +13,25 @@ namespace rdh {
 
 namespace {
 
+// Bounds-checked builds (-DRDH_CHECKED) pass every computed index through
+// chk(): an index outside its buffer sets bit `bit` of QueueCtl::violation
+// (the host then fails the render) and is replaced by 0, so the access
+// itself stays inside the buffer.  Production builds compile chk() away.
+enum CheckBit {
+  K_SLOT = 0, K_TICKET, K_JOB, K_NODE, K_PRIM, K_STACK, K_OBJECT, K_RADIANCE
+};
+template <class I>
+__device__ __forceinline__ I chk(I i, I n, CheckBit bit, QueueCtl *ctl) {
+#ifdef RDH_CHECKED
+  if (i >= n) {
+    atomicOr(&ctl->violation, 1u << bit);
+    return I(0);
+  }
+#endif
+  (void)n; (void)bit; (void)ctl;
+  return i;
+}
+
 __device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
   return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
                                    __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
@@ -264,7 +283,7 @@ __global__ void k_generate(IterArgs a, int p) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < pl.n_new; i += stride) {
     const unsigned long long w = pl.cursor + i;
     const uint32_t s = segment_of(pl.pre_new, i);
-    const uint32_t k = s * a.seg_cap + pl.cnt[s] + (i - pl.pre_new[s]);
+    const uint32_t k = chk(s * a.seg_cap + pl.cnt[s] + (i - pl.pre_new[s]), a.slots, K_SLOT, a.ctl);
     uint32_t ti = 0;
     if (!SINGLE) {
       // the wave's lanes hold consecutive work indices: one search for the
@@ -275,8 +294,9 @@ __global__ void k_generate(IterArgs a, int p) {
       ti = find_ticket(a.tickets, t_lo, t_n, w0);
       if (w >= a.tickets[ti].work_begin + a.tickets[ti].paths) ti = find_ticket(a.tickets, ti, t_n, w);
     }
+    ti = chk(ti, a.ticket_cap, K_TICKET, a.ctl);
     const DevTicket &T = a.tickets[ti];
-    const DevJob &J = a.jobs[SINGLE ? 0u : T.job];
+    const DevJob &J = a.jobs[SINGLE ? 0u : chk(T.job, a.job_cap, K_JOB, a.ctl)];
     const uint32_t pidx = (uint32_t)(w - T.work_begin);
     const uint32_t npix = J.width * J.height;
     const uint32_t pixel = pidx % npix;
@@ -381,6 +401,10 @@ void k_extend_lanes(IterArgs a, int p) {
   // job-0 fields read once (a single-job launch needs no per-ray job loads)
   const uint32_t j0_prims = J0.n_prims, j0_nodes = J0.n_nodes;
   const float j0_r0 = J0.r0_tol;
+#ifdef RDH_CHECKED
+  uint32_t cur_nodes = 0, cur_prims = 0;
+  const uint32_t rows = LDS ? a.stack_rows : (uint32_t)kStackDepth;
+#endif
   bool active = false;
   uint32_t k = 0, seg = 0, bid = kNoHit;
   int32_t node = 0;
@@ -401,12 +425,19 @@ void k_extend_lanes(IterArgs a, int p) {
       c_next += take;
       more = c_next < c_end;
       if (got) {
-        k = slot_of(pl, a.seg_cap, mine, seg);
+        k = chk(slot_of(pl, a.seg_cap, mine, seg), a.slots, K_SLOT, a.ctl);
         const F3 o = f3(q.ox[k], q.oy[k], q.oz[k]);
         const F3 d = f3(q.dx[k], q.dy[k], q.dz[k]);
-        const DevJob &J = a.jobs[SINGLE ? 0u : a.tickets[q.tb[k] >> 8].job];
+        const DevJob &J =
+            a.jobs[SINGLE ? 0u
+                          : chk(a.tickets[chk(q.tb[k] >> 8, a.ticket_cap, K_TICKET, a.ctl)].job,
+                                a.job_cap, K_JOB, a.ctl)];
         const uint32_t n_prims = SINGLE ? j0_prims : J.n_prims;
         const uint32_t n_nodes = SINGLE ? j0_nodes : J.n_nodes;
+#ifdef RDH_CHECKED
+        cur_nodes = n_nodes;
+        cur_prims = n_prims;
+#endif
         r = ray_pre(o, d);
         bt = 0.0f;
         bid = kNoHit;
@@ -448,6 +479,9 @@ void k_extend_lanes(IterArgs a, int p) {
       // one node: both child boxes, then the primitives of the hit leaf
       // children in one loop (lanes with a leaf on either side run it
       // together; the closest hit does not depend on the test order)
+#ifdef RDH_CHECKED
+      node = (int32_t)chk((uint32_t)node, cur_nodes, K_NODE, a.ctl);
+#endif
       VecF4 bx, by, bz;
       VecI4 ch;
       if (LDS) {
@@ -470,7 +504,10 @@ void k_extend_lanes(IterArgs a, int p) {
         const uint32_t f0 = leaf_first(ch.x), f1 = leaf_first(ch.y) - n0;
         for (uint32_t t = 0; t < n_leaf; ++t) {
           DevPrim pr;
-          const uint32_t j = t < n0 ? f0 + t : f1 + t;
+          uint32_t j = t < n0 ? f0 + t : f1 + t;
+#ifdef RDH_CHECKED
+          j = chk(j, cur_prims, K_PRIM, a.ctl);
+#endif
           if (LDS) {
             const VecF4 pa = lds_prims[2 * j], pb = lds_prims[2 * j + 1];
             pr.a[0] = pa.x; pr.a[1] = pa.y; pr.a[2] = pa.z; pr.a[3] = pa.w;
@@ -492,6 +529,9 @@ void k_extend_lanes(IterArgs a, int p) {
       const bool i0 = h0 && !l0, i1 = h1 && !l1;
       const bool near0 = i0 && (!i1 || t0 <= t1);
       const int32_t near_c = near0 ? ch.x : ch.y;
+#ifdef RDH_CHECKED
+      sp = (int)chk((uint32_t)sp, rows, K_STACK, a.ctl);
+#endif
       stk[sp * BLOCK] = (StackT)(near0 ? ch.y : ch.x);
       const int top = sp > 0 ? sp - 1 : 0;
       const int32_t popped = (int32_t)stk[top * BLOCK];
@@ -550,21 +590,21 @@ __global__ __launch_bounds__(BLOCK) void k_shade(IterArgs a, int p) {
     F3 no = f3(0, 0, 0), nd = no, na = no, nl = no;
     uint64_t l_slot = 0;
     if (i < pl.n) {
-      const uint32_t k = slot_of(pl, a.seg_cap, i, seg);
+      const uint32_t k = chk(slot_of(pl, a.seg_cap, i, seg), a.slots, K_SLOT, a.ctl);
       const F3 o = f3(q.ox[k], q.oy[k], q.oz[k]);
       const F3 d = f3(q.dx[k], q.dy[k], q.dz[k]);
       const F3 at = f3(q.ax[k], q.ay[k], q.az[k]);
       const F3 li = f3(q.lx[k], q.ly[k], q.lz[k]);
       const uint32_t tb = q.tb[k];
-      ti = ticket_of<SINGLE>(tb);
+      ti = chk(ticket_of<SINGLE>(tb), a.ticket_cap, K_TICKET, a.ctl);
       bounce = tb & 0xffu;
       pidx = q.pidx[k];
       const DevTicket &T = a.tickets[ti];
-      const DevJob &J = a.jobs[SINGLE ? 0u : T.job];
-      l_slot = T.l_offset + pidx;
+      const DevJob &J = a.jobs[SINGLE ? 0u : chk(T.job, a.job_cap, K_JOB, a.ctl)];
+      l_slot = chk((unsigned long long)(T.l_offset + pidx), a.l_cap, K_RADIANCE, a.ctl);
       const uint32_t id = a.hit_id[k];
       if (id != kNoHit) {
-        const DevObject ob = load_object(J.objects, id);
+        const DevObject ob = load_object(J.objects, chk(id, J.n_prims, K_OBJECT, a.ctl));
         const HitRec h = closest_hit(o, d, a.hit_t[k], ob);
         const uint32_t npix = J.width * J.height;
         BounceDraws dr;
@@ -609,7 +649,8 @@ __global__ __launch_bounds__(BLOCK) void k_shade(IterArgs a, int p) {
     }
     __syncthreads();
     if (alive && block_base != 0xffffffffu) {
-      const uint32_t k = out_seg * a.seg_cap + block_base + wave_off[wid] + lanes_below(am);
+      const uint32_t k = chk(out_seg * a.seg_cap + block_base + wave_off[wid] + lanes_below(am), a.slots,
+                             K_SLOT, a.ctl);
       nq.ox[k] = no.x; nq.oy[k] = no.y; nq.oz[k] = no.z;
       nq.dx[k] = nd.x; nq.dy[k] = nd.y; nq.dz[k] = nd.z;
       nq.ax[k] = na.x; nq.ay[k] = na.y; nq.az[k] = na.z;
@@ -634,7 +675,8 @@ __global__ __launch_bounds__(BLOCK) void k_shade(IterArgs a, int p) {
   }
   // the loop ended with a barrier: every entry of the table is final
   if (a.track_done && threadIdx.x < done.n && threadIdx.x < kDoneEntries && done.count[threadIdx.x])
-    atomicAdd(&a.ticket_done[done.ticket[threadIdx.x]], (unsigned long long)done.count[threadIdx.x]);
+    atomicAdd(&a.ticket_done[chk(done.ticket[threadIdx.x], a.ticket_cap, K_TICKET, a.ctl)],
+              (unsigned long long)done.count[threadIdx.x]);
   if (COUNT) {
     if (c_sph) atomicAdd(&a.counters->v[C_SPHERE_HIT], (unsigned long long)c_sph);
     if (c_cube) atomicAdd(&a.counters->v[C_CUBE_HIT], (unsigned long long)c_cube);
@@ -692,6 +734,7 @@ __global__ void k_set_ctl(QueueCtl *ctl, unsigned long long total, uint32_t n_ti
     ctl->count[1][s * kSegStride] = 0;
   }
   ctl->overflow = 0;
+  ctl->violation = 0;
   ctl->cursor[0] = 0;
   ctl->cursor[1] = 0;
   ctl->total = total;

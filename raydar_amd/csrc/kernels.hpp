@@ -43,6 +43,11 @@ struct IterArgs {
   uint32_t lds_bvh_bytes;           // > 0: single job whose nodes + primitives (this many
                                     // bytes) each extend workgroup copies into LDS
   uint32_t stack_rows;              // traversal stack entries per lane with lds_bvh_bytes
+  // Buffer sizes, read only by bounds-checked builds (RDH_CHECKED):
+  uint32_t slots;                   // entries of each queue array and hit array
+  uint32_t ticket_cap;              // ticket table entries
+  uint32_t job_cap;                 // job table entries
+  unsigned long long l_cap;         // radiance arena entries
 };
 
 // Dynamic LDS of the extend launch for these arguments (0: static only).

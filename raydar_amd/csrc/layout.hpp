@@ -120,7 +120,7 @@ struct QueueCtl {
   uint32_t n_tickets;       // tickets visible to generate
   uint32_t ticket_lo;       // no path before this ticket is still to be generated
   uint32_t overflow;        // set if a segment bound was ever exceeded (internal error)
-  uint32_t pad0;
+  uint32_t violation;       // bounds-checked builds: bit b = check b failed (internal error)
   unsigned long long cursor[2];  // next global work index to generate
   unsigned long long total;      // end of the work list
   unsigned long long pad1;
