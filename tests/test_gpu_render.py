@@ -228,3 +228,18 @@ def test_copy_calibration_kernel():
     with pytest.raises(ra.RaydarError) as e:
         ra.debug_copy_calibration(1000, 1)  # not a multiple of 16
     assert e.value.status == abi.RD_ERR_INVALID_ARGUMENT
+
+
+def test_wide_stack_scene_matches_oracle():
+    # 70000 objects: the BVH has more than 65535 nodes, so the traversal
+    # stack switches to 32-bit entries (plain renderer and batch alike)
+    from raydar_amd.batch import BatchRenderer, Job
+    sc = scenegen.clustered(31, 70000, 40, 24)
+    assert ra.host_bvh_info(sc, leaf=1)[1] > 65535
+    ref = oracle.render(sc, 1, 6, 3, threads=8)
+    img, acc = _gpu_acc(sc, 1, 6, 3)
+    np.testing.assert_array_equal(acc.view(np.uint32), ref.view(np.uint32))
+    job = Job(sc, 1, 6, 3)
+    with BatchRenderer(leaf=1) as br:
+        br.render([job, Job(scenegen.plain_small(0), 1, 4, 0)])
+    np.testing.assert_array_equal(job.acc.view(np.uint32), ref.view(np.uint32))
