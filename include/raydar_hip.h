@@ -201,6 +201,8 @@ rd_status rd_read_accumulator(rd_renderer *r, float *out, size_t float_capacity)
  *                 idle lanes once k or more are idle (default 48)
  *   "lds_bvh"     scenes whose BVH nodes + primitives take at most this many
  *                 bytes are traversed from LDS (default 14336; 0 = never)
+ *   "bvh4"        1 = traverse the tree collapsed to 4-wide nodes (default),
+ *                 0 = the binary tree
  *   "shade_blocks" shade workgroups per CU (0 = as many as fit)
  *   "count"       1 = collect algorithmic event counters in the next passes */
 rd_status rd_set_option(rd_renderer *r, const char *key, int64_t value);

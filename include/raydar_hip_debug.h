@@ -65,6 +65,16 @@ rd_status rd_debug_host_bvh_info(const rd_scene *scene, int leaf_size, uint32_t 
 rd_status rd_debug_host_trace(const rd_scene *scene, int leaf_size, const float *rays,
                               uint32_t n, int32_t *ids, float *ts);
 
+/* Host: the same tree collapsed to 4-wide nodes.  bvh4_info reports its node
+ * count and the stack depth its traversal needs (the kernel keeps
+ * stack4 + 3 rows) and checks that every primitive sits in exactly one leaf;
+ * trace4 walks it step for step as the extend kernel does and returns id -2
+ * for a ray whose stack would outgrow that bound. */
+rd_status rd_debug_host_bvh4_info(const rd_scene *scene, int leaf_size, uint32_t *n_nodes4,
+                                  uint32_t *stack4, char *err, size_t err_capacity);
+rd_status rd_debug_host_trace4(const rd_scene *scene, int leaf_size, const float *rays,
+                               uint32_t n, int32_t *ids, float *ts);
+
 /* Host builds of the kernels' per-ray functions (the same source compiled
  * for the CPU), with the layouts of rd_debug_scatter / rd_debug_camera_rays,
  * so the CPU test-suite can check their logic against the oracle. */

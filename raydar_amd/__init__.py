@@ -74,6 +74,10 @@ def lib():
                                              C.c_char_p, C.c_size_t]),
         "rd_debug_host_trace": (C.c_int, [sp, C.c_int, C.c_void_p, C.c_uint32, C.c_void_p,
                                           C.c_void_p]),
+        "rd_debug_host_bvh4_info": (C.c_int, [sp, C.c_int, P(C.c_uint32), P(C.c_uint32),
+                                              C.c_char_p, C.c_size_t]),
+        "rd_debug_host_trace4": (C.c_int, [sp, C.c_int, C.c_void_p, C.c_uint32, C.c_void_p,
+                                           C.c_void_p]),
         "rd_debug_host_scatter": (C.c_int, [sp, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),
         "rd_debug_host_camera_rays": (C.c_int, [sp, C.c_void_p, C.c_uint32, C.c_void_p]),
     }
@@ -302,12 +306,24 @@ def host_bvh_info(scene, leaf=4):
     return st, n.value, d.value, err.value.decode()
 
 
-def host_trace(scene, rays, leaf=4):
+def host_bvh4_info(scene, leaf=4):
+    """(status, 4-wide node count, traversal stack bound, error text)."""
+    n = C.c_uint32()
+    d = C.c_uint32()
+    err = C.create_string_buffer(256)
+    s = scene.as_rd()
+    st = lib().rd_debug_host_bvh4_info(C.byref(s), leaf, C.byref(n), C.byref(d), err, 256)
+    return st, n.value, d.value, err.value.decode()
+
+
+def host_trace(scene, rays, leaf=4, width=2):
+    """Closest hit through the binary tree (width 2) or the 4-wide tree."""
     rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 6)
     ids = np.zeros(len(rays), np.int32)
     ts = np.zeros(len(rays), np.float32)
     s = scene.as_rd()
-    check(lib().rd_debug_host_trace(C.byref(s), leaf, _ptr(rays), len(rays), _ptr(ids), _ptr(ts)))
+    fn = {2: lib().rd_debug_host_trace, 4: lib().rd_debug_host_trace4}[width]
+    check(fn(C.byref(s), leaf, _ptr(rays), len(rays), _ptr(ids), _ptr(ts)))
     return ids, ts
 
 

@@ -220,6 +220,7 @@ bool option_value(const rd_renderer *r, const std::string &k, int64_t *v) {
   else if (k == "refill") *v = o.refill;
   else if (k == "lds_bvh") *v = o.lds_bvh;
   else if (k == "shade_blocks") *v = o.shade_blocks;
+  else if (k == "bvh4") *v = o.bvh4;
   else if (k == "count") *v = o.count;
   else if (k == "kernel_timing") *v = o.kernel_timing;
   else if (k == "l_capacity") *v = (int64_t)o.l_capacity;
@@ -404,6 +405,9 @@ rd_status rd_set_option(rd_renderer *r, const char *key, int64_t v) {
   } else if (k == "refill") {
     if (v < 0 || v > 64) return r->fail(RD_ERR_INVALID_ARGUMENT, "refill must be in 0..64");
     o.refill = (int)v;
+  } else if (k == "bvh4") {
+    if (v != 0 && v != 1) return r->fail(RD_ERR_INVALID_ARGUMENT, "bvh4 must be 0 or 1");
+    o.bvh4 = (int)v;
   } else if (k == "shade_blocks") {
     if (v < 0 || v > 64) return r->fail(RD_ERR_INVALID_ARGUMENT, "shade_blocks must be in 0..64");
     o.shade_blocks = (int)v;
@@ -736,6 +740,32 @@ rd_status rd_debug_host_trace(const rd_scene *scene, int leaf, const float *rays
     return RD_ERR_INVALID_ARGUMENT;
   }
   for (uint32_t i = 0; i < n; ++i) ids[i] = rdh::host_trace(b, rays + 6ull * i, rays + 6ull * i + 3, ts + i);
+  return RD_OK;
+}
+
+rd_status rd_debug_host_bvh4_info(const rd_scene *scene, int leaf, uint32_t *n_nodes4,
+                                  uint32_t *stack4, char *err, size_t cap) {
+  if (!scene || !n_nodes4 || !stack4) return RD_ERR_INVALID_ARGUMENT;
+  rdh::BuiltScene b;
+  std::string e;
+  bool ok = rdh::build_scene(*scene, leaf, &b, &e) && rdh::check_bvh4(b, &e);
+  *n_nodes4 = (uint32_t)b.nodes4.size();
+  *stack4 = b.stack4;
+  if (err && cap) std::snprintf(err, cap, "%s", e.c_str());
+  return ok ? RD_OK : RD_ERR_INVALID_ARGUMENT;
+}
+
+rd_status rd_debug_host_trace4(const rd_scene *scene, int leaf, const float *rays, uint32_t n,
+                               int32_t *ids, float *ts) {
+  if (!scene || !rays || !ids || !ts) return RD_ERR_INVALID_ARGUMENT;
+  rdh::BuiltScene b;
+  std::string e;
+  if (!rdh::build_scene(*scene, leaf, &b, &e)) {
+    rdh::set_global_error(e);
+    return RD_ERR_INVALID_ARGUMENT;
+  }
+  for (uint32_t i = 0; i < n; ++i)
+    ids[i] = rdh::host_trace4(b, rays + 6ull * i, rays + 6ull * i + 3, ts + i);
   return RD_OK;
 }
 

@@ -221,6 +221,75 @@ struct Builder {
   }
 };
 
+// Collapses the binary tree into 4-wide nodes: a node takes its two
+// children and repeatedly replaces the inner child of largest surface area
+// by that child's two children, up to four.  Children follow their parent.
+// need = stack entries a traversal below the node can hold (an entered node
+// defers all but one of its k entered inner children).
+struct Collapser {
+  const std::vector<DevNode> &n2;
+  std::vector<DevNode4> out;
+  uint32_t depth = 0;
+  struct Child {
+    int32_t code;
+    float box[6];  // lo x, hi x, lo y, hi y, lo z, hi z
+  };
+  static double area(const Child &c) {
+    const double ex = std::max(0.0, (double)c.box[1] - c.box[0]);
+    const double ey = std::max(0.0, (double)c.box[3] - c.box[2]);
+    const double ez = std::max(0.0, (double)c.box[5] - c.box[4]);
+    return ex * ey + ey * ez + ez * ex;
+  }
+  Child child_of(int32_t node, int k) const {
+    const DevNode &m = n2[node];
+    return Child{m.child[k], {m.lo_x[k], m.hi_x[k], m.lo_y[k], m.hi_y[k], m.lo_z[k], m.hi_z[k]}};
+  }
+  int32_t collapse(int32_t b, uint32_t level, uint32_t *need) {
+    Child ch[4] = {child_of(b, 0), child_of(b, 1)};
+    int cnt = 2;
+    while (cnt < 4) {
+      int best = -1;
+      double best_area = -1.0;
+      for (int i = 0; i < cnt; ++i)
+        if (!is_leaf(ch[i].code) && area(ch[i]) > best_area) {
+          best = i;
+          best_area = area(ch[i]);
+        }
+      if (best < 0) break;
+      const int32_t m = ch[best].code;
+      ch[best] = child_of(m, 0);
+      ch[cnt++] = child_of(m, 1);
+    }
+    const uint32_t idx = (uint32_t)out.size();
+    out.emplace_back();
+    depth = std::max(depth, level);
+    uint32_t inner = 0, deepest = 0;
+    for (int i = 0; i < cnt; ++i)
+      if (!is_leaf(ch[i].code)) {
+        uint32_t nc = 0;
+        ch[i].code = collapse(ch[i].code, level + 1, &nc);
+        ++inner;
+        deepest = std::max(deepest, nc);
+      }
+    *need = inner ? inner - 1 + deepest : 0;
+    DevNode4 &nd = out[idx];
+    std::memset(&nd, 0, sizeof(nd));
+    for (int i = 0; i < 4; ++i) {
+      if (i < cnt) {
+        nd.lo_x[i] = ch[i].box[0]; nd.hi_x[i] = ch[i].box[1];
+        nd.lo_y[i] = ch[i].box[2]; nd.hi_y[i] = ch[i].box[3];
+        nd.lo_z[i] = ch[i].box[4]; nd.hi_z[i] = ch[i].box[5];
+        nd.child[i] = ch[i].code;
+      } else {
+        nd.lo_x[i] = nd.lo_y[i] = nd.lo_z[i] = std::numeric_limits<float>::infinity();
+        nd.hi_x[i] = nd.hi_y[i] = nd.hi_z[i] = -std::numeric_limits<float>::infinity();
+        nd.child[i] = leaf_code(0, 0);
+      }
+    }
+    return (int32_t)idx;
+  }
+};
+
 bool finite3(const float *v) { return std::isfinite(v[0]) && std::isfinite(v[1]) && std::isfinite(v[2]); }
 
 }  // namespace
@@ -321,6 +390,18 @@ bool build_scene(const rd_scene &scene, int leaf_size, BuiltScene *out, std::str
   }
   out->nodes = std::move(bld.nodes);
   out->depth = bld.max_depth_seen;
+  out->nodes4.clear();
+  out->depth4 = 0;
+  out->stack4 = 0;
+  if (!out->nodes.empty()) {
+    Collapser c{out->nodes, {}, 0};
+    c.out.reserve(out->nodes.size());
+    uint32_t need = 0;
+    c.collapse(0, 1, &need);
+    out->nodes4 = std::move(c.out);
+    out->depth4 = c.depth;
+    out->stack4 = need;
+  }
   out->sah_cost = bld.sah;
   out->prims.resize(n);
   for (uint32_t k = 0; k < n; ++k) {
@@ -419,6 +500,117 @@ int32_t host_trace(const BuiltScene &b, const float o[3], const float d[3], floa
       } else {
         if (sp == 0) break;
         node = stack[--sp];
+      }
+    }
+  }
+  *t_out = bid == kNoHit ? 0.f : bt;
+  return bid == kNoHit ? -1 : (int32_t)bid;
+}
+
+}  // namespace rdh
+
+namespace rdh {
+
+bool check_bvh4(const BuiltScene &b, std::string *err) {
+  const size_t n = b.prims.size();
+  if (n == 0) return b.nodes4.empty() || true;
+  std::vector<int> seen(n, 0);
+  for (size_t i = 0; i < b.nodes4.size(); ++i) {
+    const DevNode4 &nd = b.nodes4[i];
+    for (int k = 0; k < 4; ++k) {
+      const int32_t c = nd.child[k];
+      if (is_leaf(c)) {
+        const uint32_t f = leaf_first(c), cnt = leaf_count(c);
+        if (f + cnt > n || cnt > (uint32_t)kMaxLeafSize) {
+          *err = "4-wide leaf out of range";
+          return false;
+        }
+        for (uint32_t j = f; j < f + cnt; ++j) seen[j]++;
+      } else if (c <= (int32_t)i || (size_t)c >= b.nodes4.size()) {
+        *err = "4-wide child index not after its parent";
+        return false;
+      }
+    }
+  }
+  for (size_t j = 0; j < n; ++j)
+    if (seen[j] != 1) {
+      *err = "primitive " + std::to_string(j) + " referenced " + std::to_string(seen[j]) +
+             " times in the 4-wide tree";
+      return false;
+    }
+  return true;
+}
+
+// One step of the 4-wide traversal exactly as the extend kernel takes it:
+// the entered inner children are sorted by entry distance with a
+// five-comparator network, the three farther candidates are written above
+// the stack top every step (kept as many as were entered, minus one) and the
+// top entry is read every step.  Rows needed: stack4 + 3.
+int32_t host_trace4(const BuiltScene &b, const float o[3], const float d[3], float *t_out) {
+  const RayPre r = ray_pre(f3(o[0], o[1], o[2]), f3(d[0], d[1], d[2]));
+  float bt = 0.f;
+  uint32_t bid = kNoHit;
+  const bool finite = std::isfinite(d[0]) && std::isfinite(d[1]) && std::isfinite(d[2]) &&
+                      std::isfinite(o[0]) && std::isfinite(o[1]) && std::isfinite(o[2]);
+  if (!finite || b.nodes4.empty()) {
+    for (const DevPrim &p : b.prims) test_prim(r, p, &bt, &bid);
+  } else {
+    const SlabRay s = slab_ray(r, ray_tolerance(r, b.r0));
+    const uint32_t rows = b.stack4 + 3;
+    std::vector<int32_t> stk(rows, 0);
+    uint32_t sp = 0;
+    int32_t node = 0;
+    float tcull = INFINITY;
+    for (;;) {
+      const DevNode4 &nd = b.nodes4[node];
+      float t[4];
+      bool h[4], l[4], in[4];
+      uint32_t cnt_leaf[4], n_leaf = 0;
+      for (int c = 0; c < 4; ++c) {
+        h[c] = slab_hit(s, nd.lo_x[c], nd.hi_x[c], nd.lo_y[c], nd.hi_y[c], nd.lo_z[c], nd.hi_z[c],
+                        tcull, &t[c]);
+        l[c] = h[c] && is_leaf(nd.child[c]);
+        in[c] = h[c] && !l[c];
+        cnt_leaf[c] = l[c] ? leaf_count(nd.child[c]) : 0u;
+        n_leaf += cnt_leaf[c];
+      }
+      if (n_leaf) {
+        for (int c = 0; c < 4; ++c)
+          for (uint32_t j = 0; j < cnt_leaf[c]; ++j)
+            test_prim(r, b.prims[leaf_first(nd.child[c]) + j], &bt, &bid);
+        tcull = cull_limit(bt, bid);
+      }
+      float key[4];
+      int32_t code[4];
+      for (int c = 0; c < 4; ++c) {
+        key[c] = in[c] ? t[c] : INFINITY;
+        code[c] = nd.child[c];
+      }
+      auto cmpx = [&](int a, int c) {
+        if (key[c] < key[a]) {
+          std::swap(key[a], key[c]);
+          std::swap(code[a], code[c]);
+        }
+      };
+      cmpx(0, 1); cmpx(2, 3); cmpx(0, 2); cmpx(1, 3); cmpx(1, 2);
+      const uint32_t cnt = (uint32_t)in[0] + in[1] + in[2] + in[3];
+      if (sp + 2 >= rows) {
+        *t_out = 0.f;
+        return -2;  // the stack bound is wrong
+      }
+      stk[sp] = cnt == 4 ? code[3] : cnt == 3 ? code[2] : code[1];
+      stk[sp + 1] = cnt == 4 ? code[2] : code[1];
+      stk[sp + 2] = code[1];
+      const uint32_t top = sp > 0 ? sp - 1 : 0;
+      const int32_t popped = stk[top];
+      if (cnt > 0) {
+        node = code[0];
+        sp += cnt - 1;
+      } else if (sp > 0) {
+        node = popped;
+        sp = top;
+      } else {
+        break;
       }
     }
   }

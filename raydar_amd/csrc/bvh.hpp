@@ -18,6 +18,9 @@ namespace rdh {
 
 struct BuiltScene {
   std::vector<DevNode> nodes;
+  std::vector<DevNode4> nodes4;    // the same tree, collapsed to 4-wide nodes
+  uint32_t depth4 = 0;             // levels of nodes4
+  uint32_t stack4 = 0;             // traversal stack entries nodes4 can need
   std::vector<DevPrim> prims;      // BVH leaf order
   std::vector<DevObject> objects;  // original order
   uint32_t depth = 0;              // max node depth (root = 1)
@@ -36,5 +39,11 @@ bool check_bvh(const BuiltScene &b, std::string *err);
 // Host reference traversal of the device BVH format, used by the CPU tests of
 // the builder: same culling rule as the extend kernel, exact hit tests.
 int32_t host_trace(const BuiltScene &b, const float o[3], const float d[3], float *t_out);
+
+// The same over the 4-wide nodes (children visited nearest first).
+int32_t host_trace4(const BuiltScene &b, const float o[3], const float d[3], float *t_out);
+
+// Structural invariants of nodes4 (as check_bvh) and its stack bound.
+bool check_bvh4(const BuiltScene &b, std::string *err);
 
 }  // namespace rdh

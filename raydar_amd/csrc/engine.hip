@@ -143,17 +143,23 @@ bool Engine::ensure_tickets(uint32_t n, std::string *err) {
 bool Engine::upload_scene(const BuiltScene &b, DeviceScene *out, std::string *err) {
   free_scene(out);
   out->n_nodes = (uint32_t)b.nodes.size();
+  out->n_nodes4 = (uint32_t)b.nodes4.size();
+  out->stack4 = b.stack4;
   out->n_prims = (uint32_t)b.prims.size();
   out->r0 = b.r0;
   out->depth = b.depth;
   const size_t nb = sizeof(DevNode) * b.nodes.size(), pb = sizeof(DevPrim) * b.prims.size(),
-               ob = sizeof(DevObject) * b.objects.size();
-  out->bytes = nb + pb + ob;
+               ob = sizeof(DevObject) * b.objects.size(), n4 = sizeof(DevNode4) * b.nodes4.size();
+  out->bytes = nb + pb + ob + n4;
   if (nb && !check(hipMalloc(&out->nodes, nb), "hipMalloc(bvh)", err)) return false;
+  if (n4 && !check(hipMalloc(&out->nodes4, n4), "hipMalloc(bvh4)", err)) return false;
   if (pb && !check(hipMalloc(&out->prims, pb), "hipMalloc(prims)", err)) return false;
   if (ob && !check(hipMalloc(&out->objects, ob), "hipMalloc(objects)", err)) return false;
   if (nb && !check(hipMemcpyAsync(out->nodes, b.nodes.data(), nb, hipMemcpyHostToDevice, stream),
                    "upload bvh", err))
+    return false;
+  if (n4 && !check(hipMemcpyAsync(out->nodes4, b.nodes4.data(), n4, hipMemcpyHostToDevice, stream),
+                   "upload bvh4", err))
     return false;
   if (pb && !check(hipMemcpyAsync(out->prims, b.prims.data(), pb, hipMemcpyHostToDevice, stream),
                    "upload prims", err))
@@ -167,6 +173,7 @@ bool Engine::upload_scene(const BuiltScene &b, DeviceScene *out, std::string *er
 
 void Engine::free_scene(DeviceScene *s) {
   if (s->nodes) (void)hipFree(s->nodes);
+  if (s->nodes4) (void)hipFree(s->nodes4);
   if (s->prims) (void)hipFree(s->prims);
   if (s->objects) (void)hipFree(s->objects);
   *s = DeviceScene();
@@ -190,6 +197,7 @@ IterArgs Engine::iter_args(const DevJob *d_jobs) const {
   a.refill = opt.refill;
   a.lds_bvh_bytes = 0;
   a.stack_rows = 0;
+  a.bvh4 = 0;
   a.slots = (uint32_t)kSegments * seg_cap;
   a.ticket_cap = ticket_alloc;
   a.job_cap = 1;
@@ -208,7 +216,8 @@ LaunchShape Engine::shape(int which, const IterArgs *a) const {
   } else {
     const bool lanes = which == 1 && opt.refill > 0;
     const uint32_t dyn = (which == 1 && a) ? extend_dynamic_lds(*a, s.block) : 0;
-    int per_cu = std::max(1, occupancy_blocks(which, s.block, lanes, dyn ? a->lds_bvh_bytes : 0, dyn));
+    const bool w4 = which == 1 && a && a->bvh4;
+    int per_cu = std::max(1, occupancy_blocks(which, s.block, lanes, dyn ? a->lds_bvh_bytes : 0, dyn, w4));
     if (which == 2 && opt.shade_blocks > 0) per_cu = std::min(per_cu, opt.shade_blocks);
     s.grid = num_cus * per_cu;
   }
@@ -280,12 +289,17 @@ bool Engine::run_tickets(const DevJob *d_jobs, std::vector<DevTicket> tickets,
     for (const DevJob &j : host_jobs)
       if (j.n_nodes > kNarrowStackNodes) a.wide_stack = 1;
     if (a.single_job && !a.wide_stack && !opt.brute_force && opt.refill > 0) {
-      // a small scene is traversed from LDS, with a stack as deep as the tree
+      // One scene: the 4-wide tree when its stack bound fits the kernel's
+      // stack; a small scene is traversed from LDS, with a stack as deep as
+      // the traversal can need.
       const DevJob &j = host_jobs[0];
-      const uint64_t bytes = (uint64_t)sizeof(DevNode) * j.n_nodes + (uint64_t)sizeof(DevPrim) * j.n_prims;
-      if (j.n_nodes > 0 && bytes <= opt.lds_bvh && j.depth + 1 <= (uint32_t)kStackDepth) {
+      a.bvh4 = opt.bvh4 && j.n_nodes4 > 0 && j.stack4 + 3 <= (uint32_t)kStackDepth ? 1 : 0;
+      const uint64_t nb = a.bvh4 ? (uint64_t)sizeof(DevNode4) * j.n_nodes4 : (uint64_t)sizeof(DevNode) * j.n_nodes;
+      const uint64_t bytes = nb + (uint64_t)sizeof(DevPrim) * j.n_prims;
+      const uint32_t rows = a.bvh4 ? j.stack4 + 3 : j.depth + 1;
+      if (j.n_nodes > 0 && bytes <= opt.lds_bvh && rows <= (uint32_t)kStackDepth) {
         a.lds_bvh_bytes = (uint32_t)bytes;
-        a.stack_rows = j.depth + 1;
+        a.stack_rows = rows;
       }
     }
     int parity = 0;
@@ -374,6 +388,9 @@ void fill_job(const rd_scene &s, const DeviceScene &ds, float *acc, uint32_t max
   }
   j->r0_tol = ds.r0;
   j->depth = ds.depth;
+  j->nodes4 = ds.nodes4;
+  j->n_nodes4 = ds.n_nodes4;
+  j->stack4 = ds.stack4;
 }
 
 }  // namespace rdh

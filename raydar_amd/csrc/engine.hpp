@@ -22,6 +22,8 @@ namespace rdh {
 // Device copy of a BuiltScene.
 struct DeviceScene {
   DevNode *nodes = nullptr;
+  DevNode4 *nodes4 = nullptr;
+  uint32_t n_nodes4 = 0, stack4 = 0;
   DevPrim *prims = nullptr;
   DevObject *objects = nullptr;
   uint32_t n_nodes = 0, n_prims = 0;
@@ -40,6 +42,7 @@ struct EngineOptions {
   uint32_t lds_bvh = 14336;       // single scenes with at most this many BVH bytes
                                   // are traversed from LDS (0: never)
   int shade_blocks = 0;           // shade workgroups per CU (0: as many as fit)
+  int bvh4 = 1;                   // single scenes: traverse the 4-wide tree
   int leaf = 1;                   // BVH leaf size; 1 measured fastest for one scene
   int count = 0;
   int kernel_timing = 0;

@@ -349,6 +349,18 @@ void k_extend(IterArgs a, int p) {
   }
 }
 
+// Comparator of the 4-wide traversal's sorting network: (ka, ca) gets the
+// smaller key (a strictly smaller kb swaps, as the host traversal does).
+__device__ __forceinline__ void order_pair(float &ka, int32_t &ca, float &kb, int32_t &cb) {
+  const bool sw = kb < ka;
+  const float k = sw ? kb : ka;
+  const int32_t c = sw ? cb : ca;
+  kb = sw ? ka : kb;
+  cb = sw ? ca : cb;
+  ka = k;
+  ca = c;
+}
+
 // Persistent-lane closest hit.  Rays differ widely in traversal length, so
 // a wave that traces 64 rays to completion keeps most lanes idle while its
 // longest ray finishes.  Here each wave owns a contiguous range of queue
@@ -362,7 +374,13 @@ void k_extend(IterArgs a, int p) {
 // LDS = true (single job, small scene): the workgroup first copies the BVH
 // nodes and primitives into LDS and traverses them there; the stack then
 // has only as many rows as the tree has levels, both in dynamic LDS.
-template <int BLOCK, bool COUNT, bool SINGLE, class StackT, bool LDS>
+//
+// W4 = true (single job): the 4-wide tree (DevJob::nodes4).  A step tests
+// four child boxes, sorts the entered inner children by entry distance with
+// five comparators, writes the three farther candidates above the stack top
+// (kept as many as were entered, minus one) and reads the top entry; the
+// stack needs DevJob::stack4 + 3 rows.
+template <int BLOCK, bool COUNT, bool SINGLE, class StackT, bool LDS, bool W4>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(RDH_EXTEND_WAVES, 8)))
 void k_extend_lanes(IterArgs a, int p) {
   __shared__ StackT static_stack[LDS ? 1 : kStackDepth * BLOCK];
@@ -375,10 +393,11 @@ void k_extend_lanes(IterArgs a, int p) {
     const uint32_t stack_bytes = (a.stack_rows * BLOCK * (uint32_t)sizeof(StackT) + 15u) & ~15u;
     stack = reinterpret_cast<StackT *>(dynamic_lds);
     lds_nodes = reinterpret_cast<VecF4 *>(dynamic_lds + stack_bytes);
-    lds_prims = lds_nodes + 4 * J.n_nodes;
-    GlobalF4 *gn = (GlobalF4 *)J.nodes;
+    const uint32_t node_vecs = W4 ? 8 * J.n_nodes4 : 4 * J.n_nodes;
+    lds_prims = lds_nodes + node_vecs;
+    GlobalF4 *gn = W4 ? (GlobalF4 *)J.nodes4 : (GlobalF4 *)J.nodes;
     GlobalF4 *gp = (GlobalF4 *)J.prims;
-    for (uint32_t i = threadIdx.x; i < 4 * J.n_nodes; i += BLOCK) lds_nodes[i] = gn[i];
+    for (uint32_t i = threadIdx.x; i < node_vecs; i += BLOCK) lds_nodes[i] = gn[i];
     for (uint32_t i = threadIdx.x; i < 2 * J.n_prims; i += BLOCK) lds_prims[i] = gp[i];
   }
   make_plan(a, p, pl);  // its barriers also publish the LDS copy
@@ -397,6 +416,7 @@ void k_extend_lanes(IterArgs a, int p) {
 
   const DevJob &J0 = a.jobs[0];
   const DevNode *nodes = J0.nodes;
+  const DevNode4 *nodes4 = J0.nodes4;
   const DevPrim *prims = J0.prims;
   // job-0 fields read once (a single-job launch needs no per-ray job loads)
   const uint32_t j0_prims = J0.n_prims, j0_nodes = J0.n_nodes;
@@ -404,6 +424,7 @@ void k_extend_lanes(IterArgs a, int p) {
 #ifdef RDH_CHECKED
   uint32_t cur_nodes = 0, cur_prims = 0;
   const uint32_t rows = LDS ? a.stack_rows : (uint32_t)kStackDepth;
+  const uint32_t j0_nodes4 = J0.n_nodes4;
 #endif
   bool active = false;
   uint32_t k = 0, seg = 0, bid = kNoHit;
@@ -435,7 +456,7 @@ void k_extend_lanes(IterArgs a, int p) {
         const uint32_t n_prims = SINGLE ? j0_prims : J.n_prims;
         const uint32_t n_nodes = SINGLE ? j0_nodes : J.n_nodes;
 #ifdef RDH_CHECKED
-        cur_nodes = n_nodes;
+        cur_nodes = W4 ? j0_nodes4 : n_nodes;
         cur_prims = n_prims;
 #endif
         r = ray_pre(o, d);
@@ -476,76 +497,164 @@ void k_extend_lanes(IterArgs a, int p) {
       continue;
     }
     if (active) {
-      // one node: both child boxes, then the primitives of the hit leaf
-      // children in one loop (lanes with a leaf on either side run it
-      // together; the closest hit does not depend on the test order)
+      if constexpr (W4) {
 #ifdef RDH_CHECKED
-      node = (int32_t)chk((uint32_t)node, cur_nodes, K_NODE, a.ctl);
+        node = (int32_t)chk((uint32_t)node, cur_nodes, K_NODE, a.ctl);
 #endif
-      VecF4 bx, by, bz;
-      VecI4 ch;
-      if (LDS) {
-        const VecF4 *np = lds_nodes + 4 * node;
-        bx = np[0]; by = np[1]; bz = np[2];
-        ch = __builtin_bit_cast(VecI4, np[3]);
-      } else {
-        GlobalF4 *np = (GlobalF4 *)(nodes + node);
-        bx = np[0]; by = np[1]; bz = np[2];
-        ch = ((GlobalI4 *)np)[3];
-      }
-      float t0, t1;
-      const bool h0 = slab_hit(s, bx.x, bx.z, by.x, by.z, bz.x, bz.z, tcull, &t0);
-      const bool h1 = slab_hit(s, bx.y, bx.w, by.y, by.w, bz.y, bz.w, tcull, &t1);
-      if (COUNT) tc.box += 2;
-      const bool l0 = h0 && is_leaf(ch.x), l1 = h1 && is_leaf(ch.y);
-      const uint32_t n0 = l0 ? leaf_count(ch.x) : 0u;
-      const uint32_t n_leaf = n0 + (l1 ? leaf_count(ch.y) : 0u);
-      if (n_leaf) {
-        const uint32_t f0 = leaf_first(ch.x), f1 = leaf_first(ch.y) - n0;
-        for (uint32_t t = 0; t < n_leaf; ++t) {
-          DevPrim pr;
-          uint32_t j = t < n0 ? f0 + t : f1 + t;
-#ifdef RDH_CHECKED
-          j = chk(j, cur_prims, K_PRIM, a.ctl);
-#endif
-          if (LDS) {
-            const VecF4 pa = lds_prims[2 * j], pb = lds_prims[2 * j + 1];
-            pr.a[0] = pa.x; pr.a[1] = pa.y; pr.a[2] = pa.z; pr.a[3] = pa.w;
-            pr.b[0] = pb.x; pr.b[1] = pb.y; pr.b[2] = pb.z; pr.b[3] = pb.w;
-          } else {
-            load_prim(prims, j, &pr);
-          }
-          if (COUNT) {
-            if (fbits(pr.b[3]) & kTagCube) tc.cube++;
-            else tc.sphere++;
-          }
-          test_prim(r, pr, &bt, &bid);
+        VecF4 lx, hx, ly, hy, lz, hz;
+        VecI4 ch;
+        if (LDS) {
+          const VecF4 *np = lds_nodes + 8 * node;
+          lx = np[0]; hx = np[1]; ly = np[2]; hy = np[3]; lz = np[4]; hz = np[5];
+          ch = __builtin_bit_cast(VecI4, np[6]);
+        } else {
+          GlobalF4 *np = (GlobalF4 *)(nodes4 + node);
+          lx = np[0]; hx = np[1]; ly = np[2]; hy = np[3]; lz = np[4]; hz = np[5];
+          ch = ((GlobalI4 *)np)[6];
         }
-        tcull = cull_limit(bt, bid);  // changes only when a primitive is tested
-      }
-      // next node without branches: the far child is written above the
-      // stack top every step (kept only when both children are entered;
-      // slot sp < depth is in bounds) and the top entry is read every step
-      const bool i0 = h0 && !l0, i1 = h1 && !l1;
-      const bool near0 = i0 && (!i1 || t0 <= t1);
-      const int32_t near_c = near0 ? ch.x : ch.y;
+        float t0, t1, t2, t3;
+        const bool h0 = slab_hit(s, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, tcull, &t0);
+        const bool h1 = slab_hit(s, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, tcull, &t1);
+        const bool h2 = slab_hit(s, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, tcull, &t2);
+        const bool h3 = slab_hit(s, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, tcull, &t3);
+        if (COUNT) {  // empty slots (leaf code with no primitives) are not boxes of the tree
+          const int32_t e = leaf_code(0, 0);
+          tc.box += (ch.x != e) + (ch.y != e) + (ch.z != e) + (ch.w != e);
+        }
+        const bool l0 = h0 && is_leaf(ch.x), l1 = h1 && is_leaf(ch.y);
+        const bool l2 = h2 && is_leaf(ch.z), l3 = h3 && is_leaf(ch.w);
+        const uint32_t n0 = l0 ? leaf_count(ch.x) : 0u;
+        const uint32_t p1 = n0 + (l1 ? leaf_count(ch.y) : 0u);
+        const uint32_t p2 = p1 + (l2 ? leaf_count(ch.z) : 0u);
+        const uint32_t n_leaf = p2 + (l3 ? leaf_count(ch.w) : 0u);
+        if (n_leaf) {
+          const uint32_t f0 = leaf_first(ch.x), f1 = leaf_first(ch.y) - n0;
+          const uint32_t f2 = leaf_first(ch.z) - p1, f3 = leaf_first(ch.w) - p2;
+          for (uint32_t t = 0; t < n_leaf; ++t) {
+            DevPrim pr;
+            uint32_t j = t < n0 ? f0 + t : t < p1 ? f1 + t : t < p2 ? f2 + t : f3 + t;
 #ifdef RDH_CHECKED
-      sp = (int)chk((uint32_t)sp, rows, K_STACK, a.ctl);
+            j = chk(j, cur_prims, K_PRIM, a.ctl);
 #endif
-      stk[sp * BLOCK] = (StackT)(near0 ? ch.y : ch.x);
-      const int top = sp > 0 ? sp - 1 : 0;
-      const int32_t popped = (int32_t)stk[top * BLOCK];
-      if (i0 || i1) {
-        node = near_c;
-        sp += (i0 && i1) ? 1 : 0;
-      } else if (sp > 0) {
-        node = popped;
-        sp = top;
+            if (LDS) {
+              const VecF4 pa = lds_prims[2 * j], pb = lds_prims[2 * j + 1];
+              pr.a[0] = pa.x; pr.a[1] = pa.y; pr.a[2] = pa.z; pr.a[3] = pa.w;
+              pr.b[0] = pb.x; pr.b[1] = pb.y; pr.b[2] = pb.z; pr.b[3] = pb.w;
+            } else {
+              load_prim(prims, j, &pr);
+            }
+            if (COUNT) {
+              if (fbits(pr.b[3]) & kTagCube) tc.cube++;
+              else tc.sphere++;
+            }
+            test_prim(r, pr, &bt, &bid);
+          }
+          tcull = cull_limit(bt, bid);
+        }
+        // entered inner children in order of entry distance
+        const bool i0 = h0 && !l0, i1 = h1 && !l1, i2 = h2 && !l2, i3 = h3 && !l3;
+        float k0 = i0 ? t0 : INFINITY, k1 = i1 ? t1 : INFINITY;
+        float k2 = i2 ? t2 : INFINITY, k3 = i3 ? t3 : INFINITY;
+        int32_t c0 = ch.x, c1 = ch.y, c2 = ch.z, c3 = ch.w;
+        order_pair(k0, c0, k1, c1);
+        order_pair(k2, c2, k3, c3);
+        order_pair(k0, c0, k2, c2);
+        order_pair(k1, c1, k3, c3);
+        order_pair(k1, c1, k2, c2);
+        const int cnt = (int)i0 + (int)i1 + (int)i2 + (int)i3;
+#ifdef RDH_CHECKED
+        sp = (int)chk((uint32_t)sp + 2u, rows, K_STACK, a.ctl) - 2;
+        if (sp < 0) sp = 0;
+#endif
+        stk[sp * BLOCK] = (StackT)(cnt == 4 ? c3 : cnt == 3 ? c2 : c1);
+        stk[(sp + 1) * BLOCK] = (StackT)(cnt == 4 ? c2 : c1);
+        stk[(sp + 2) * BLOCK] = (StackT)c1;
+        const int top = sp > 0 ? sp - 1 : 0;
+        const int32_t popped = (int32_t)stk[top * BLOCK];
+        if (cnt > 0) {
+          node = c0;
+          sp += cnt - 1;
+        } else if (sp > 0) {
+          node = popped;
+          sp = top;
+        } else {
+          a.hit_t[k] = bid == kNoHit ? 0.0f : bt;
+          a.hit_id[k] = bid;
+          active = false;
+          ++rays;
+        }
       } else {
-        a.hit_t[k] = bid == kNoHit ? 0.0f : bt;
-        a.hit_id[k] = bid;
-        active = false;
-        ++rays;
+        // one node: both child boxes, then the primitives of the hit leaf
+        // children in one loop (lanes with a leaf on either side run it
+        // together; the closest hit does not depend on the test order)
+  #ifdef RDH_CHECKED
+        node = (int32_t)chk((uint32_t)node, cur_nodes, K_NODE, a.ctl);
+  #endif
+        VecF4 bx, by, bz;
+        VecI4 ch;
+        if (LDS) {
+          const VecF4 *np = lds_nodes + 4 * node;
+          bx = np[0]; by = np[1]; bz = np[2];
+          ch = __builtin_bit_cast(VecI4, np[3]);
+        } else {
+          GlobalF4 *np = (GlobalF4 *)(nodes + node);
+          bx = np[0]; by = np[1]; bz = np[2];
+          ch = ((GlobalI4 *)np)[3];
+        }
+        float t0, t1;
+        const bool h0 = slab_hit(s, bx.x, bx.z, by.x, by.z, bz.x, bz.z, tcull, &t0);
+        const bool h1 = slab_hit(s, bx.y, bx.w, by.y, by.w, bz.y, bz.w, tcull, &t1);
+        if (COUNT) tc.box += 2;
+        const bool l0 = h0 && is_leaf(ch.x), l1 = h1 && is_leaf(ch.y);
+        const uint32_t n0 = l0 ? leaf_count(ch.x) : 0u;
+        const uint32_t n_leaf = n0 + (l1 ? leaf_count(ch.y) : 0u);
+        if (n_leaf) {
+          const uint32_t f0 = leaf_first(ch.x), f1 = leaf_first(ch.y) - n0;
+          for (uint32_t t = 0; t < n_leaf; ++t) {
+            DevPrim pr;
+            uint32_t j = t < n0 ? f0 + t : f1 + t;
+  #ifdef RDH_CHECKED
+            j = chk(j, cur_prims, K_PRIM, a.ctl);
+  #endif
+            if (LDS) {
+              const VecF4 pa = lds_prims[2 * j], pb = lds_prims[2 * j + 1];
+              pr.a[0] = pa.x; pr.a[1] = pa.y; pr.a[2] = pa.z; pr.a[3] = pa.w;
+              pr.b[0] = pb.x; pr.b[1] = pb.y; pr.b[2] = pb.z; pr.b[3] = pb.w;
+            } else {
+              load_prim(prims, j, &pr);
+            }
+            if (COUNT) {
+              if (fbits(pr.b[3]) & kTagCube) tc.cube++;
+              else tc.sphere++;
+            }
+            test_prim(r, pr, &bt, &bid);
+          }
+          tcull = cull_limit(bt, bid);  // changes only when a primitive is tested
+        }
+        // next node without branches: the far child is written above the
+        // stack top every step (kept only when both children are entered;
+        // slot sp < depth is in bounds) and the top entry is read every step
+        const bool i0 = h0 && !l0, i1 = h1 && !l1;
+        const bool near0 = i0 && (!i1 || t0 <= t1);
+        const int32_t near_c = near0 ? ch.x : ch.y;
+  #ifdef RDH_CHECKED
+        sp = (int)chk((uint32_t)sp, rows, K_STACK, a.ctl);
+  #endif
+        stk[sp * BLOCK] = (StackT)(near0 ? ch.y : ch.x);
+        const int top = sp > 0 ? sp - 1 : 0;
+        const int32_t popped = (int32_t)stk[top * BLOCK];
+        if (i0 || i1) {
+          node = near_c;
+          sp += (i0 && i1) ? 1 : 0;
+        } else if (sp > 0) {
+          node = popped;
+          sp = top;
+        } else {
+          a.hit_t[k] = bid == kNoHit ? 0.0f : bt;
+          a.hit_id[k] = bid;
+          active = false;
+          ++rays;
+        }
       }
     }
   }
@@ -848,14 +957,17 @@ uint32_t blocks_for(uint64_t n, uint32_t block) { return (uint32_t)((n + block -
 // Kernel instance for a launch: block size, counting build, single-job
 // specialisation and stack width are template parameters.
 template <bool COUNT, bool SINGLE>
-static const void *extend_fn(int block, bool wide, bool lanes, bool lds) {
-#define RDH_EXT(B)                                                                      \
-  if (block == B) {                                                                     \
-    if (lanes && lds && SINGLE && !wide)                                                \
-      return (const void *)k_extend_lanes<B, COUNT, true, uint16_t, true>;              \
-    if (lanes)                                                                          \
-      return wide ? (const void *)k_extend_lanes<B, COUNT, SINGLE, int32_t, false>      \
-                  : (const void *)k_extend_lanes<B, COUNT, SINGLE, uint16_t, false>;    \
+static const void *extend_fn(int block, bool wide, bool lanes, bool lds, bool w4) {
+#define RDH_EXT(B)                                                                           \
+  if (block == B) {                                                                          \
+    if (lanes && SINGLE && !wide && w4)                                                      \
+      return lds ? (const void *)k_extend_lanes<B, COUNT, true, uint16_t, true, true>        \
+                 : (const void *)k_extend_lanes<B, COUNT, true, uint16_t, false, true>;      \
+    if (lanes && lds && SINGLE && !wide)                                                     \
+      return (const void *)k_extend_lanes<B, COUNT, true, uint16_t, true, false>;            \
+    if (lanes)                                                                               \
+      return wide ? (const void *)k_extend_lanes<B, COUNT, SINGLE, int32_t, false, false>    \
+                  : (const void *)k_extend_lanes<B, COUNT, SINGLE, uint16_t, false, false>;  \
     return wide ? (const void *)k_extend<B, COUNT, SINGLE, int32_t>              \
                 : (const void *)k_extend<B, COUNT, SINGLE, uint16_t>;            \
   }
@@ -873,25 +985,25 @@ static const void *shade_fn(int block) {
   return nullptr;
 }
 static const void *kernel_fn(int which, int block, bool count, bool single, bool wide,
-                             bool lanes = false, bool lds = false) {
+                             bool lanes = false, bool lds = false, bool w4 = false) {
   if (which == 0) {
     if (count) return single ? (const void *)k_generate<true, true> : (const void *)k_generate<true, false>;
     return single ? (const void *)k_generate<false, true> : (const void *)k_generate<false, false>;
   }
   if (which == 1) {
     if (count)
-      return single ? extend_fn<true, true>(block, wide, lanes, lds)
-                    : extend_fn<true, false>(block, wide, lanes, lds);
-    return single ? extend_fn<false, true>(block, wide, lanes, lds)
-                  : extend_fn<false, false>(block, wide, lanes, lds);
+      return single ? extend_fn<true, true>(block, wide, lanes, lds, w4)
+                    : extend_fn<true, false>(block, wide, lanes, lds, w4);
+    return single ? extend_fn<false, true>(block, wide, lanes, lds, w4)
+                  : extend_fn<false, false>(block, wide, lanes, lds, w4);
   }
   if (count) return single ? shade_fn<true, true>(block) : shade_fn<true, false>(block);
   return single ? shade_fn<false, true>(block) : shade_fn<false, false>(block);
 }
 
 int occupancy_blocks(int which, int block, bool lanes, uint32_t lds_bvh_bytes,
-                     uint32_t dynamic_lds) {
-  const void *fn = kernel_fn(which, block, false, true, false, lanes, lds_bvh_bytes > 0);
+                     uint32_t dynamic_lds, bool w4) {
+  const void *fn = kernel_fn(which, block, false, true, false, lanes, lds_bvh_bytes > 0, w4);
   int n = 0;
   if (!fn || hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, block, dynamic_lds) != hipSuccess) {
     (void)hipGetLastError();
@@ -909,7 +1021,7 @@ static hipError_t launch_stage(int which, const IterArgs &a, int parity, const L
                                bool count, hipStream_t st) {
   const uint32_t dyn = which == 1 ? extend_dynamic_lds(a, s.block) : 0;
   const void *fn = kernel_fn(which, s.block, count, a.single_job != 0, a.wide_stack != 0,
-                             a.refill > 0, dyn > 0);
+                             a.refill > 0, dyn > 0, a.bvh4 != 0);
   if (!fn) return hipErrorInvalidValue;
   IterArgs args = a;
   int par = parity;

@@ -43,6 +43,7 @@ struct IterArgs {
   uint32_t lds_bvh_bytes;           // > 0: single job whose nodes + primitives (this many
                                     // bytes) each extend workgroup copies into LDS
   uint32_t stack_rows;              // traversal stack entries per lane with lds_bvh_bytes
+  int bvh4;                         // single job: traverse DevJob::nodes4
   // Buffer sizes, read only by bounds-checked builds (RDH_CHECKED):
   uint32_t slots;                   // entries of each queue array and hit array
   uint32_t ticket_cap;              // ticket table entries
@@ -70,7 +71,7 @@ hipError_t launch_tonemap(const DevJob *job, uint32_t n_pixels, float spp, uchar
                           DevCounters *counters, bool count, int grid, hipStream_t st);
 // Workgroups of kernel `which` (0 generate, 1 extend, 2 shade) that fit on one CU.
 int occupancy_blocks(int which, int block, bool lanes = false, uint32_t lds_bvh_bytes = 0,
-                     uint32_t dynamic_lds = 0);
+                     uint32_t dynamic_lds = 0, bool w4 = false);
 
 hipError_t launch_set_ctl(QueueCtl *ctl, unsigned long long total, uint32_t n_tickets,
                           hipStream_t st);

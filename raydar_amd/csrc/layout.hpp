@@ -48,6 +48,18 @@ struct alignas(16) DevNode {
 };
 static_assert(sizeof(DevNode) == 64, "DevNode must be 64 bytes");
 
+// 4-wide node (the binary tree collapsed): the boxes of up to four children,
+// axis-major with child k in lane k, and their codes.  An unused child is an
+// empty leaf, leaf_code(0, 0): hitting its box tests nothing.
+struct alignas(16) DevNode4 {
+  float lo_x[4], hi_x[4];
+  float lo_y[4], hi_y[4];
+  float lo_z[4], hi_z[4];
+  int32_t child[4];
+  uint32_t pad[4];
+};
+static_assert(sizeof(DevNode4) == 128, "DevNode4 must be 128 bytes");
+
 // Primitive tag in DevPrim::b.w: original object id | kind << 31.
 struct alignas(16) DevPrim {
   // sphere: a = (cx, cy, cz, dot(C,C)), b = (r*r, 0, 0, tag)
@@ -84,7 +96,9 @@ struct DevJob {
   float top[4], bottom[4], solid[4];
   float r0_tol;             // |o|_1 above which a ray needs the dynamic box margin
   uint32_t depth;           // BVH levels (a traversal stack needs fewer entries)
-  float pad_[2];
+  uint32_t n_nodes4;        // the same tree as 4-wide nodes
+  uint32_t stack4;          // stack entries a 4-wide traversal can hold
+  const DevNode4 *nodes4;
 };
 
 struct DevTicket {

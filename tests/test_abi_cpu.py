@@ -127,9 +127,14 @@ def _random_scene(seed, n):
 @pytest.mark.parametrize("leaf", [1, 2, 4, 8])
 @pytest.mark.parametrize("n", [1, 2, 7, 100, 2048])
 def test_bvh_invariants(leaf, n):
-    st, nodes, depth, err = ra.host_bvh_info(_random_scene(n, n), leaf)
+    sc = _random_scene(n, n)
+    st, nodes, depth, err = ra.host_bvh_info(sc, leaf)
     assert st == abi.RD_OK, err
     assert nodes >= 1 and depth <= 38
+    st, nodes4, stack4, err = ra.host_bvh4_info(sc, leaf)
+    assert st == abi.RD_OK, err
+    # every 4-wide node absorbs at least one binary node
+    assert 1 <= nodes4 <= nodes and stack4 <= 3 * depth
 
 
 def test_bvh_degenerate_inputs():
@@ -139,23 +144,27 @@ def test_bvh_degenerate_inputs():
     assert st == abi.RD_OK, err
     ids, _ = ra.host_trace(same, np.array([[1, 2, 9, 0, 0, -1]], np.float32), 1)
     assert ids[0] == 0  # 300 coincident spheres: the lowest index wins
+    ids, _ = ra.host_trace(same, np.array([[1, 2, 9, 0, 0, -1]], np.float32), 1, 4)
+    assert ids[0] == 0
     empty = Scene(same.camera, sky_world(), np.zeros(0, abi.OBJECT_DTYPE))
     ids, _ = ra.host_trace(empty, np.array([[0, 0, 9, 0, 0, -1]], np.float32))
     assert ids[0] == -1
 
 
+@pytest.mark.parametrize("width", [2, 4])
 @pytest.mark.parametrize("scene_name", ["plain_small", "plain_1080p", "clustered_1024", "benchmark"])
 @pytest.mark.parametrize("leaf", [1, 4, 8])
-def test_bvh_traversal_matches_linear_scan(scene_name, leaf):
+def test_bvh_traversal_matches_linear_scan(scene_name, leaf, width):
     """The BVH culling rule (padded boxes, cull_limit) never drops the scan's
-    answer, phantom hits of the FP32 sphere test included."""
+    answer, phantom hits of the FP32 sphere test included; the 4-wide walk
+    (sorted children, fixed stack writes) also stays within its stack bound."""
     sc = {"plain_small": lambda: scenegen.plain_small(0),
           "plain_1080p": lambda: scenegen.plain_1080p(1),
           "clustered_1024": lambda: scenegen.clustered(2, 1024, 320, 180),
           "benchmark": lambda: Scene.load(os.path.join(GOLDEN, "benchmark.rscn"))}[scene_name]()
     rays = raygen.mixed(sc, 2500, seed=leaf)
     ids_o, ts_o, _, _ = oracle.trace(sc, rays)
-    ids_h, ts_h = ra.host_trace(sc, rays, leaf)
+    ids_h, ts_h = ra.host_trace(sc, rays, leaf, width)
     np.testing.assert_array_equal(ids_h, ids_o)
     hit = ids_o >= 0
     np.testing.assert_array_equal(ts_h[hit].view(np.uint32), ts_o[hit].view(np.uint32))

@@ -40,7 +40,8 @@ def _gpu_acc(scene, spp, bounces, seed, **opts):
                                   dict(leaf=8, block=128, queue=65536), dict(traversal=1),
                                   dict(refill=0), dict(refill=1, grid=7), dict(refill=64, leaf=2),
                                   dict(refill=0, block=64, queue=4096), dict(lds_bvh=0),
-                                  dict(lds_bvh=0, refill=32, block=128)])
+                                  dict(lds_bvh=0, refill=32, block=128), dict(bvh4=0),
+                                  dict(bvh4=0, lds_bvh=0), dict(bvh4=0, leaf=8, refill=16)])
 def test_plain_small_matches_oracle(opts):
     sc = scenegen.plain_small(0)
     img, acc = _gpu_acc(sc, 16, 12, 5, **opts)
@@ -52,14 +53,27 @@ def test_plain_small_matches_oracle(opts):
     np.testing.assert_array_equal(img.reshape(-1), oracle.tonemap(ref, 16))
 
 
+@pytest.mark.parametrize("bvh4", [0, 1])
 @pytest.mark.parametrize("lds_bvh", [0, 14336, 1 << 17])
-def test_lds_resident_bvh_matches_oracle(lds_bvh):
-    # 97 objects (9.3 KB of nodes + primitives): traversed from LDS unless disabled
+def test_lds_resident_bvh_matches_oracle(lds_bvh, bvh4):
+    # 97 objects (at most 9.3 KB of nodes + primitives): traversed from LDS
+    # unless disabled, through the binary or the 4-wide tree
     sc = scenegen.plain_1080p(1, 96, 54)
-    img, acc = _gpu_acc(sc, 3, 12, 21, lds_bvh=lds_bvh)
+    img, acc = _gpu_acc(sc, 3, 12, 21, lds_bvh=lds_bvh, bvh4=bvh4)
     ref = oracle.render(sc, 3, 12, 21)
     np.testing.assert_array_equal(acc.view(np.uint32), ref.view(np.uint32))
     np.testing.assert_array_equal(img.reshape(-1), oracle.tonemap(ref, 3))
+
+
+@pytest.mark.parametrize("bvh4", [0, 1])
+@pytest.mark.parametrize("leaf", [1, 4])
+def test_clustered_scene_both_trees_match_oracle(leaf, bvh4):
+    # 4096 clustered objects: a deep tree walked from global memory
+    sc = scenegen.clustered(2, 4096, 160, 90)
+    assert ra.host_bvh4_info(sc, leaf)[0] == abi.RD_OK
+    img, acc = _gpu_acc(sc, 2, 12, 8, leaf=leaf, bvh4=bvh4)
+    ref = oracle.render(sc, 2, 12, 8, threads=8)
+    np.testing.assert_array_equal(acc.view(np.uint32), ref.view(np.uint32))
 
 
 def test_default_rscn_matches_oracle():
@@ -243,3 +257,24 @@ def test_wide_stack_scene_matches_oracle():
     with BatchRenderer(leaf=1) as br:
         br.render([job, Job(scenegen.plain_small(0), 1, 4, 0)])
     np.testing.assert_array_equal(job.acc.view(np.uint32), ref.view(np.uint32))
+
+
+def test_event_counts_agree_between_trees():
+    # the counting pass behind the roofline figure: the two trees give the
+    # same paths, so every per-path event agrees and only the traversal
+    # work (boxes and primitives tested) differs
+    sc = scenegen.clustered(2, 4096, 160, 90)
+    counts = {}
+    for bvh4 in (0, 1):
+        with ra.HipRenderer(max_sample_count=2, max_bounces=12, seed=8, bvh4=bvh4) as r:
+            r.reset_counters()
+            r.set_option("count", 1)
+            r.render_frame(sc)
+            counts[bvh4] = r.counters()
+    a, b = counts[0], counts[1]
+    assert a["camera_rays"] == b["camera_rays"] == sc.width * sc.height * 2
+    for k in ("sphere_hits", "cube_hits", "glass_scatters", "misses", "accumulates"):
+        assert a[k] == b[k], k
+    assert a["sphere_hits"] + a["cube_hits"] + a["misses"] == a["segments"] == b["segments"]
+    for k in ("box_tests", "sphere_tests", "cube_tests"):
+        assert a[k] > 0 and b[k] > 0, k
