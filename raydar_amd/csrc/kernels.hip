@@ -304,8 +304,8 @@ __global__ void k_generate(IterArgs a, int p) {
     camera_ray(J, pixel % J.width, pixel / J.width, &o, &d);
     q.ox[k] = o.x; q.oy[k] = o.y; q.oz[k] = o.z;
     q.dx[k] = d.x; q.dy[k] = d.y; q.dz[k] = d.z;
-    q.ax[k] = 1.0f; q.ay[k] = 1.0f; q.az[k] = 1.0f;
-    q.lx[k] = 0.0f; q.ly[k] = 0.0f; q.lz[k] = 0.0f;
+    // attenuation and light of a camera path are (1,1,1) and 0 (cpu.rs:253-254):
+    // shade uses the constants at bounce 0, so they are not stored
     q.tb[k] = ti << 8;
     q.pidx[k] = pidx;
     ++made;
@@ -702,11 +702,14 @@ __global__ __launch_bounds__(BLOCK) void k_shade(IterArgs a, int p) {
       const uint32_t k = chk(slot_of(pl, a.seg_cap, i, seg), a.slots, K_SLOT, a.ctl);
       const F3 o = f3(q.ox[k], q.oy[k], q.oz[k]);
       const F3 d = f3(q.dx[k], q.dy[k], q.dz[k]);
-      const F3 at = f3(q.ax[k], q.ay[k], q.az[k]);
-      const F3 li = f3(q.lx[k], q.ly[k], q.lz[k]);
       const uint32_t tb = q.tb[k];
       ti = chk(ticket_of<SINGLE>(tb), a.ticket_cap, K_TICKET, a.ctl);
       bounce = tb & 0xffu;
+      F3 at = f3(1.0f, 1.0f, 1.0f), li = f3(0.0f, 0.0f, 0.0f);
+      if (bounce != 0) {
+        at = f3(q.ax[k], q.ay[k], q.az[k]);
+        li = f3(q.lx[k], q.ly[k], q.lz[k]);
+      }
       pidx = q.pidx[k];
       const DevTicket &T = a.tickets[ti];
       const DevJob &J = a.jobs[SINGLE ? 0u : chk(T.job, a.job_cap, K_JOB, a.ctl)];
