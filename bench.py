@@ -252,7 +252,10 @@ def main():
     step()
     r.set_option("count", 0)
     cnt = r.counters()
-    # kernel-timing pass (hipEvents around every launch on the library's stream)
+    # kernel-timing pass: hipEvents around every launch on the stream it runs
+    # on, each launch waited for before the next, so every kernel is timed
+    # alone (in the timed steps the pipes' kernels overlap)
+    pipes_used = r.get_option("pipes_used")
     r.reset_counters()
     r.set_option("kernel_timing", 1)
     step()
@@ -286,7 +289,9 @@ def main():
                         f"{len(sc.objects)} objects, seed {job.seed}",
             "paths_per_step": paths_per_step,
             "renderer": "plain per-scene HIP renderer (wavefront: generate/extend/shade, SAH BVH)",
-            "options": {k: r.get_option(k) for k in ("block", "queue", "leaf", "grid", "traversal")},
+            "options": {k: r.get_option(k) for k in ("block", "queue", "leaf", "grid", "traversal",
+                                                      "bvh4", "pipes")},
+            "pipes_used": pipes_used,
         },
         "mrays_per_s": round(rays_per_step / step_s / 1e6, 3),
         "rays_per_path": round(rays_per_step / paths_per_step, 4),
@@ -306,6 +311,7 @@ def main():
             "flop_per_launch": round(ext_flop_per_launch),
             "avg_launch_ms": round(ext_avg_s * 1e3, 5),
             "launches": ext_n,
+            "timing": "each launch timed alone (kernel-timing pass, launches serialized)",
         },
         "kernel_ms_per_step": {k: round(v[0], 3) for k, v in kt.items()},
         "kernel_launches_per_step": {k: v[1] for k, v in kt.items()},

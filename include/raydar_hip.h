@@ -203,6 +203,11 @@ rd_status rd_read_accumulator(rd_renderer *r, float *out, size_t float_capacity)
  *                 bytes are traversed from LDS (default 14336; 0 = never)
  *   "bvh4"        1 = traverse the tree collapsed to 4-wide nodes (default),
  *                 0 = the binary tree
+ *   "pipes"       a pass of k >= 2 samples is divided by samples over up to
+ *                 min(pipes, k) independent streams whose kernels run
+ *                 concurrently, each taking at least queue/2 paths
+ *                 (1..4, default 2; 1 = one stream); read-only
+ *                 "pipes_used" reports the last pass's count
  *   "shade_blocks" shade workgroups per CU (0 = as many as fit)
  *   "count"       1 = collect algorithmic event counters in the next passes */
 rd_status rd_set_option(rd_renderer *r, const char *key, int64_t value);

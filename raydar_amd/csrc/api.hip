@@ -221,6 +221,8 @@ bool option_value(const rd_renderer *r, const std::string &k, int64_t *v) {
   else if (k == "lds_bvh") *v = o.lds_bvh;
   else if (k == "shade_blocks") *v = o.shade_blocks;
   else if (k == "bvh4") *v = o.bvh4;
+  else if (k == "pipes") *v = o.pipes;
+  else if (k == "pipes_used") *v = r->eng.pipes_used;
   else if (k == "count") *v = o.count;
   else if (k == "kernel_timing") *v = o.kernel_timing;
   else if (k == "l_capacity") *v = (int64_t)o.l_capacity;
@@ -408,6 +410,9 @@ rd_status rd_set_option(rd_renderer *r, const char *key, int64_t v) {
   } else if (k == "bvh4") {
     if (v != 0 && v != 1) return r->fail(RD_ERR_INVALID_ARGUMENT, "bvh4 must be 0 or 1");
     o.bvh4 = (int)v;
+  } else if (k == "pipes") {
+    if (v < 1 || v > 4) return r->fail(RD_ERR_INVALID_ARGUMENT, "pipes must be in 1..4");
+    o.pipes = (int)v;
   } else if (k == "shade_blocks") {
     if (v < 0 || v > 64) return r->fail(RD_ERR_INVALID_ARGUMENT, "shade_blocks must be in 0..64");
     o.shade_blocks = (int)v;

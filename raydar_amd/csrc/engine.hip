@@ -274,62 +274,146 @@ bool Engine::run_tickets(const DevJob *d_jobs, std::vector<DevTicket> tickets,
     return false;
   }
   if (!traced.empty()) {
-    const uint32_t n = (uint32_t)traced.size();
-    if (!ensure_tickets(n, err)) return false;
-    if (!check(hipMemcpyAsync(d_tickets, traced.data(), sizeof(DevTicket) * n,
-                              hipMemcpyHostToDevice, stream),
-               "upload tickets", err))
-      return false;
-    if (!check(hipMemsetAsync(d_ticket_done, 0, 8ull * n, stream), "hipMemset", err)) return false;
-    if (!check(launch_set_ctl(ctl, w, n, stream), "set ctl", err)) return false;
-    IterArgs a = iter_args(d_jobs);
-    a.single_job = host_jobs.size() == 1 ? 1 : 0;
-    a.job_cap = (uint32_t)host_jobs.size();
-    a.track_done = 0;  // completion is known from the drained queue
-    for (const DevJob &j : host_jobs)
-      if (j.n_nodes > kNarrowStackNodes) a.wide_stack = 1;
-    if (a.single_job && !a.wide_stack && !opt.brute_force && opt.refill > 0) {
-      // One scene: the 4-wide tree when its stack bound fits the kernel's
-      // stack; a small scene is traversed from LDS, with a stack as deep as
-      // the traversal can need.
-      const DevJob &j = host_jobs[0];
-      a.bvh4 = opt.bvh4 && j.n_nodes4 > 0 && j.stack4 + 3 <= (uint32_t)kStackDepth ? 1 : 0;
-      const uint64_t nb = a.bvh4 ? (uint64_t)sizeof(DevNode4) * j.n_nodes4 : (uint64_t)sizeof(DevNode) * j.n_nodes;
-      const uint64_t bytes = nb + (uint64_t)sizeof(DevPrim) * j.n_prims;
-      const uint32_t rows = a.bvh4 ? j.stack4 + 3 : j.depth + 1;
-      if (j.n_nodes > 0 && bytes <= opt.lds_bvh && rows <= (uint32_t)kStackDepth) {
-        a.lds_bvh_bytes = (uint32_t)bytes;
-        a.stack_rows = rows;
+    // Pipes: a one-job pass is divided by samples (pipe k traces samples
+    // [s_k, s_k+1) into their own radiance slots), anything else runs whole.
+    // Each pipe must still get half a queue of paths: below that the extra
+    // launches cost more than the overlap gains (measured break-even between
+    // 4M and 8M paths per pass at the default 8M-path queue).
+    std::vector<std::vector<DevTicket>> parts;
+    int np = 1;
+    if (traced.size() == 1 && host_jobs.size() == 1) {
+      const uint64_t by_size = traced[0].paths / std::max<uint64_t>(1, q_alloc / 2);
+      np = (int)std::min<uint64_t>({(uint64_t)std::max(1, opt.pipes), traced[0].spp, by_size});
+      np = std::max(np, 1);
+    }
+    pipes_used = np;
+    if (np <= 1) {
+      parts.push_back(traced);
+    } else {
+      const DevTicket &t = traced[0];
+      const uint64_t npix = t.paths / t.spp;
+      uint32_t s0 = 0;
+      for (int k = 0; k < np; ++k) {
+        const uint32_t s1 = (uint32_t)((uint64_t)t.spp * (k + 1) / np);
+        DevTicket p = t;
+        p.work_begin = 0;
+        p.l_offset = t.l_offset + s0 * npix;
+        p.sample_base = t.sample_base + s0;
+        p.spp = s1 - s0;
+        p.paths = (uint32_t)(p.spp * npix);
+        parts.push_back({p});
+        s0 = s1;
       }
     }
-    int parity = 0;
-    const uint64_t cap_iters = (w / q_alloc + 2) * (uint64_t)(max_bounces + 1) + 64;
-    uint64_t it = 0;
+    while ((int)aux_.size() < np - 1) {
+      auto e = std::make_unique<Engine>();
+      e->device = device;
+      if (!e->init(err)) return false;
+      aux_.push_back(std::move(e));
+    }
+    struct Pipe {
+      Engine *e;
+      IterArgs a;
+      uint64_t w;
+      int parity;
+      uint64_t it, cap_iters;
+      bool done;
+    };
+    std::vector<Pipe> pipes;
+    for (int k = 0; k < np; ++k) {
+      Engine *e = k == 0 ? this : aux_[k - 1].get();
+      if (k > 0) {
+        e->opt = opt;
+        e->opt.l_capacity = 0;  // radiance goes to this engine's arena
+        e->opt.pipes = 1;
+        if (!e->ensure_buffers(err)) return false;
+      }
+      const std::vector<DevTicket> &tk = parts[k];
+      const uint32_t n = (uint32_t)tk.size();
+      uint64_t wk = 0;
+      for (const DevTicket &t : tk) wk += t.paths;
+      if (!e->ensure_tickets(n, err)) return false;
+      if (!check(hipMemcpyAsync(e->d_tickets, tk.data(), sizeof(DevTicket) * n,
+                                hipMemcpyHostToDevice, e->stream),
+                 "upload tickets", err))
+        return false;
+      if (!check(hipMemsetAsync(e->d_ticket_done, 0, 8ull * n, e->stream), "hipMemset", err))
+        return false;
+      if (!check(launch_set_ctl(e->ctl, wk, n, e->stream), "set ctl", err)) return false;
+      IterArgs a = e->iter_args(d_jobs);
+      a.L = L;
+      a.l_cap = l_alloc;
+      a.counters = counters;
+      a.single_job = host_jobs.size() == 1 ? 1 : 0;
+      a.job_cap = (uint32_t)host_jobs.size();
+      a.track_done = 0;  // completion is known from the drained queue
+      for (const DevJob &j : host_jobs)
+        if (j.n_nodes > kNarrowStackNodes) a.wide_stack = 1;
+      if (a.single_job && !a.wide_stack && !opt.brute_force && opt.refill > 0) {
+        // One scene: the 4-wide tree when its stack bound fits the kernel's
+        // stack; a small scene is traversed from LDS, with a stack as deep as
+        // the traversal can need.
+        const DevJob &j = host_jobs[0];
+        a.bvh4 = opt.bvh4 && j.n_nodes4 > 0 && j.stack4 + 3 <= (uint32_t)kStackDepth ? 1 : 0;
+        const uint64_t nb = a.bvh4 ? (uint64_t)sizeof(DevNode4) * j.n_nodes4 : (uint64_t)sizeof(DevNode) * j.n_nodes;
+        const uint64_t bytes = nb + (uint64_t)sizeof(DevPrim) * j.n_prims;
+        const uint32_t rows = a.bvh4 ? j.stack4 + 3 : j.depth + 1;
+        if (j.n_nodes > 0 && bytes <= opt.lds_bvh && rows <= (uint32_t)kStackDepth) {
+          a.lds_bvh_bytes = (uint32_t)bytes;
+          a.stack_rows = rows;
+        }
+      }
+      const uint64_t cap_iters = (wk / e->q_alloc + 2) * (uint64_t)(max_bounces + 1) + 64;
+      pipes.push_back(Pipe{e, a, wk, 0, 0, cap_iters, false});
+    }
+    const int group = std::max(1, opt.sync_every);
     for (;;) {
-      const int group = std::max(1, opt.sync_every);
-      for (int g = 0; g < group; ++g) {
-        if (!launch_iteration(a, parity, err)) return false;
-        parity ^= 1;
-        ++it;
+      for (int g = 0; g < group; ++g)
+        for (Pipe &p : pipes) {
+          if (p.done) continue;
+          if (!p.e->launch_iteration(p.a, p.parity, err)) return false;
+          p.parity ^= 1;
+          ++p.it;
+        }
+      for (Pipe &p : pipes)
+        if (!p.done && !check(hipMemcpyAsync(p.e->ctl_host, p.e->ctl, sizeof(QueueCtl),
+                                             hipMemcpyDeviceToHost, p.e->stream),
+                              "read queue state", err))
+          return false;
+      bool all = true;
+      for (Pipe &p : pipes) {
+        if (p.done) continue;
+        if (!check(hipStreamSynchronize(p.e->stream), "iteration", err)) return false;
+        const QueueCtl &c = *p.e->ctl_host;
+        if (c.overflow) {
+          *err = "path queue segment overflow (internal error)";
+          return false;
+        }
+        if (c.violation) {
+          *err = "device bounds check failed, mask " + std::to_string(c.violation) +
+                 " (internal error)";
+          return false;
+        }
+        if (queued(c, p.parity) == 0 && c.cursor[p.parity] == p.w) {
+          p.done = true;
+        } else if (p.it > p.cap_iters) {
+          *err = "wavefront loop made no progress (internal error)";
+          return false;
+        }
+        all = all && p.done;
       }
-      if (!check(hipMemcpyAsync(ctl_host, ctl, sizeof(QueueCtl), hipMemcpyDeviceToHost, stream),
-                 "read queue state", err))
-        return false;
-      if (!check(hipStreamSynchronize(stream), "iteration", err)) return false;
-      if (ctl_host->overflow) {
-        *err = "path queue segment overflow (internal error)";
-        return false;
+      if (all) break;
+    }
+    for (int k = 1; k < np; ++k) {  // the pipes' kernel statistics are this engine's
+      Engine &e = *aux_[k - 1];
+      for (int i = 0; i < 5; ++i) {
+        kernel_ms[i] += e.kernel_ms[i];
+        kernel_launches[i] += e.kernel_launches[i];
+        e.kernel_ms[i] = 0;
+        e.kernel_launches[i] = 0;
       }
-      if (ctl_host->violation) {
-        *err = "device bounds check failed, mask " + std::to_string(ctl_host->violation) +
-               " (internal error)";
-        return false;
-      }
-      if (queued(*ctl_host, parity) == 0 && ctl_host->cursor[parity] == w) break;
-      if (it > cap_iters) {
-        *err = "wavefront loop made no progress (internal error)";
-        return false;
-      }
+      iterations += e.iterations;
+      e.iterations = 0;
     }
   }
   // Accumulate every ticket in order: per pixel, samples in sample order.

@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -47,6 +48,8 @@ struct EngineOptions {
   int count = 0;
   int kernel_timing = 0;
   int sync_every = 4;             // iterations between completion checks
+  int pipes = 2;                  // one-scene passes: independent streams the
+                                  // samples are divided over (1..4)
 };
 
 class Engine {
@@ -62,7 +65,12 @@ class Engine {
 
   // Runs one batch of tickets to completion (all paths finished), then
   // accumulates every ticket into its job's accumulator in ticket order.
-  // `jobs` is the device job table (already uploaded).
+  // `jobs` is the device job table (already uploaded).  A single-job pass of
+  // two or more samples is divided by samples over `opt.pipes` pipes: this
+  // engine and auxiliary engines with their own stream, queues and tickets,
+  // writing this engine's radiance arena.  The pipes' kernels run
+  // concurrently, so one pipe's closest-hit work (ALU-bound) overlaps
+  // another's shading (memory-bound).
   bool run_tickets(const DevJob *d_jobs, std::vector<DevTicket> tickets,
                    const std::vector<DevJob> &host_jobs, std::string *err);
 
@@ -76,6 +84,7 @@ class Engine {
   double kernel_ms[5] = {0, 0, 0, 0, 0};
   uint64_t kernel_launches[5] = {0, 0, 0, 0, 0};
   uint64_t iterations = 0;
+  int pipes_used = 0;  // pipes of the last traced pass
 
   // buffers (public so the server can drive iterations itself)
   PathQueue q[2] = {};
@@ -105,6 +114,7 @@ class Engine {
                                                              hipStream_t),
              const IterArgs &a, int parity);
   std::vector<hipEvent_t> ev_pool_;
+  std::vector<std::unique_ptr<Engine>> aux_;  // pipes 1.. of run_tickets
 };
 
 // Message returned by rd_last_error(NULL) (calls with no renderer handle).

@@ -41,7 +41,9 @@ def _gpu_acc(scene, spp, bounces, seed, **opts):
                                   dict(refill=0), dict(refill=1, grid=7), dict(refill=64, leaf=2),
                                   dict(refill=0, block=64, queue=4096), dict(lds_bvh=0),
                                   dict(lds_bvh=0, refill=32, block=128), dict(bvh4=0),
-                                  dict(bvh4=0, lds_bvh=0), dict(bvh4=0, leaf=8, refill=16)])
+                                  dict(bvh4=0, lds_bvh=0), dict(bvh4=0, leaf=8, refill=16),
+                                  dict(pipes=1), dict(pipes=3, queue=65536), dict(pipes=4, queue=4096),
+                                  dict(pipes=4, sync_every=1, lds_bvh=0, queue=1 << 16)])
 def test_plain_small_matches_oracle(opts):
     sc = scenegen.plain_small(0)
     img, acc = _gpu_acc(sc, 16, 12, 5, **opts)
@@ -278,3 +280,27 @@ def test_event_counts_agree_between_trees():
     assert a["sphere_hits"] + a["cube_hits"] + a["misses"] == a["segments"] == b["segments"]
     for k in ("box_tests", "sphere_tests", "cube_tests"):
         assert a[k] > 0 and b[k] > 0, k
+
+
+@pytest.mark.parametrize("spp,pipes,queue,used", [(3, 2, 4096, 2), (5, 4, 4096, 4), (2, 4, 4096, 2),
+                                                   (1, 2, 4096, 1), (5, 4, 1 << 16, 2),
+                                                   (5, 2, 1 << 23, 1)])
+def test_pass_divided_over_pipes_matches_oracle(spp, pipes, queue, used):
+    # a pass of spp samples runs on up to min(pipes, spp) streams, each
+    # tracing a contiguous run of samples into its own radiance slots and
+    # each given at least half a queue of paths (160x90 = 14400 per sample)
+    sc = scenegen.plain_1080p(1, 160, 90)
+    with ra.HipRenderer(max_sample_count=spp, max_bounces=12, seed=4, pipes=pipes,
+                        queue=queue) as r:
+        img = r.render_frame(sc)
+        acc = r.accumulator()
+        assert r.get_option("pipes_used") == used
+    ref = oracle.render(sc, spp, 12, 4)
+    np.testing.assert_array_equal(acc.view(np.uint32), ref.view(np.uint32))
+    np.testing.assert_array_equal(img.reshape(-1), oracle.tonemap(ref, spp))
+    with ra.HipRenderer(max_sample_count=spp, max_bounces=12, seed=4) as r:
+        assert r.get_option("pipes") == 2
+        for bad in (0, 5):
+            with pytest.raises(ra.RaydarError) as e:
+                r.set_option("pipes", bad)
+            assert e.value.status == abi.RD_ERR_INVALID_ARGUMENT
