@@ -111,7 +111,7 @@ def batched_vs_plain(autotune=True):
         dt_b = time.perf_counter() - t0
         out["batched"] = {"seconds": round(dt_b, 4), "mpaths_per_s": round(paths / dt_b / 1e6, 2),
                           "iterations": st["iterations"], "graph_launches": st["graph_launches"],
-                          "options": {k: br.get_option(k) for k in ("block", "queue", "leaf")}}
+                          "options": {k: br.get_option(k) for k in ("block", "queue", "leaf", "pipes")}}
     with ra.HipRenderer() as r:
         r.max_sample_count = 1
         r.render_frame(specs[0].scene)  # warm-up

@@ -55,7 +55,11 @@ void rd_batch_destroy(rd_batch *b);
 /* Keys: "block" (64/128/256), "queue" (paths in flight), "leaf" (1..8, default 2),
  * "grid" (workgroups, 0 = auto), "refill" (0..64, as rd_set_option),
  * "graph" (1 = replay a captured hipGraph),
- * "l_capacity" (radiance slots), "iters_per_pump" (even, >= 2). */
+ * "l_capacity" (radiance slots), "iters_per_pump" (even, >= 2),
+ * "pipes" (1..4, default 2; settable until the first render): independent
+ * engines, each with its own stream, queues and radiance ring; every job is
+ * given whole to the one with the fewest pending paths and the host pumps
+ * them in turn, so their kernels overlap on the GPU. */
 rd_status rd_batch_set_option(rd_batch *b, const char *key, int64_t value);
 rd_status rd_batch_get_option(const rd_batch *b, const char *key, int64_t *value);
 /* Render all jobs to completion in one mixed wavefront. */
@@ -108,6 +112,8 @@ typedef struct rd_server_stats {
   uint32_t jobs_resident, jobs_queued, max_resident;
 } rd_server_stats;
 
+/* The server runs two pipes (see rd_batch_set_option); hbm_budget_bytes
+ * covers both engines' buffers and every admitted job. */
 rd_status rd_server_create(const rd_server_config *cfg, rd_server **out);
 /* Stops accepting work, finishes every submitted job, frees the device. */
 void rd_server_destroy(rd_server *s);

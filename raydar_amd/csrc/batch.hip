@@ -67,6 +67,15 @@ struct SJob {
   Clock::time_point t_submit, t_admit, t_done;
 };
 
+// Device-memory account of one or more streamers: one budget for jobs plus
+// the engines' own buffers.
+struct HbmPool {
+  uint64_t budget = 200ull << 30;
+  uint64_t fixed = 0;  // engine buffers of every ready streamer
+  uint64_t in_use = 0, peak = 0;
+  uint32_t resident = 0, max_resident = 0;
+};
+
 struct LAlloc {
   uint64_t off, size;
   uint32_t ticket;
@@ -79,14 +88,13 @@ class Streamer {
  public:
   Engine eng;
   std::string err;
-  uint64_t budget = 200ull << 30;
+  HbmPool own_pool;
+  HbmPool *pool = &own_pool;  // shared when the streamer is a group member
   int use_graph = 1;
   int iters_per_pump = 2;
   uint32_t default_pass = 0;
   // statistics
   uint64_t paths_done = 0, graph_launches = 0, pumps = 0;
-  uint64_t hbm_in_use = 0, hbm_peak = 0;
-  uint32_t max_resident = 0;
 
   ~Streamer() { teardown(); }
 
@@ -112,9 +120,12 @@ class Streamer {
     h_jobs_.assign(kJobSlots, DevJob{});
     tickets_.reserve(kTicketCap);  // never reallocates: async uploads read from it
     fixed_bytes_ = engine_bytes();
+    pool->fixed += fixed_bytes_;
     ready_ = true;
     return eng.check(hipStreamSynchronize(eng.stream), "init", &err);
   }
+
+  uint64_t engine_bytes_if_ready() const { return ready_ ? fixed_bytes_ : 0; }
 
   uint64_t engine_bytes() const {
     return (uint64_t)kSegments * eng.seg_cap * (2 * 56 + 8) + eng.l_alloc * 16 + kTicketCap * 40ull +
@@ -159,7 +170,7 @@ class Streamer {
     return j;
   }
 
-  bool fits_alone(const SJob &j) const { return j.bytes + fixed_bytes_ <= budget; }
+  bool fits_alone(const SJob &j) const { return j.bytes + pool->fixed <= pool->budget; }
 
   void enqueue(std::unique_ptr<SJob> j) {
     queued_.push_back(j.get());
@@ -176,7 +187,11 @@ class Streamer {
   void forget(uint64_t id) { owned_.erase(id); }
 
   // One round of the host loop.  Returns false on a device error.
-  bool pump() {
+  bool pump() { return launch() && harvest(); }
+
+  // First half of a round: admit jobs, append tickets and enqueue
+  // iterations on this streamer's stream without waiting for them.
+  bool launch() {
     if (!ensure_ready()) return false;
     ++pumps;
     // Launch-shape options may have changed since the queues were sized;
@@ -188,7 +203,16 @@ class Streamer {
     if (work) {
       if (!run_iterations()) return false;
     }
-    return harvest();
+    return true;
+  }
+
+  // Paths of queued and resident jobs not yet accumulated.
+  uint64_t pending_paths() const {
+    uint64_t n = 0;
+    for (const SJob *j : queued_) n += (uint64_t)j->W * j->H * j->spp;
+    for (const SJob *j : resident_)
+      n += (uint64_t)j->W * j->H * (j->spp - std::min(j->spp, j->passes_done * j->pass_spp));
+    return n;
   }
 
   uint64_t iterations() const { return eng.iterations; }
@@ -241,7 +265,8 @@ class Streamer {
     while (!queued_.empty()) {
       SJob *j = queued_.front();
       if (free_slots_.empty()) break;
-      if (hbm_in_use + j->bytes + fixed_bytes_ > budget && !resident_.empty()) break;
+      // wait for room unless nothing is resident (a job that fits alone always runs)
+      if (pool->in_use + j->bytes + pool->fixed > pool->budget && pool->resident > 0) break;
       queued_.pop_front();
       j->slot = free_slots_.back();
       free_slots_.pop_back();
@@ -262,9 +287,10 @@ class Streamer {
       j->state = 1;
       j->t_admit = Clock::now();
       resident_.push_back(j);
-      hbm_in_use += j->bytes;
-      hbm_peak = std::max(hbm_peak, hbm_in_use);
-      max_resident = std::max<uint32_t>(max_resident, (uint32_t)resident_.size());
+      pool->in_use += j->bytes;
+      pool->peak = std::max(pool->peak, pool->in_use);
+      pool->resident++;
+      pool->max_resident = std::max(pool->max_resident, pool->resident);
     }
     return true;
   }
@@ -399,6 +425,9 @@ class Streamer {
     return true;
   }
 
+  // Second half of a round: wait for the stream, read the ticket progress
+  // and accumulate every finished pass in pass order.
+ public:
   bool harvest() {
     const uint32_t n = uploaded_;
     if (!eng.check(hipMemcpyAsync(eng.ctl_host, eng.ctl, sizeof(QueueCtl), hipMemcpyDeviceToHost,
@@ -491,9 +520,160 @@ class Streamer {
         j->state = 2;
         release_job(j);
         free_slots_.push_back(j->slot);
-        hbm_in_use -= j->bytes;
+        pool->in_use -= j->bytes;
+        pool->resident--;
         resident_.erase(std::find(resident_.begin(), resident_.end(), j));
       }
+    }
+    return true;
+  }
+};
+
+// Streamers ("pipes") pumped in turn by one host thread.  Each owns its
+// stream, queues, work list, radiance ring and job table; every job is given
+// whole to the member with the fewest pending paths.  A round harvests a
+// member's previous iterations and launches its next ones before moving to
+// the next member, so while the host waits on one stream the other streams
+// have iterations queued, and their closest-hit (ALU-bound) and shading
+// (memory-bound) kernels share the CUs.
+class StreamerGroup {
+ public:
+  std::vector<std::unique_ptr<Streamer>> m;
+  std::string err;
+  uint64_t pumps = 0;
+
+  bool init(int n) {
+    m.clear();
+    launched_.clear();
+    return grow(n);
+  }
+
+  // Members can be added or removed only before any of them has rendered.
+  bool unused() const {
+    for (const auto &s : m)
+      if (s->eng.q_alloc || !s->idle()) return false;
+    return true;
+  }
+  bool resize(int n) {
+    if ((int)m.size() > n) {
+      m.resize(n);
+      launched_.resize(n);
+      pool_.fixed = 0;
+      for (const auto &s : m) pool_.fixed += s->engine_bytes_if_ready();
+      return true;
+    }
+    return grow(n);
+  }
+
+  Streamer &first() { return *m[0]; }
+  const Streamer &first() const { return *m[0]; }
+  template <class F> void each(F f) {
+    for (auto &s : m) f(*s);
+  }
+
+  // One HBM budget for all members (jobs plus every member's own buffers).
+  void set_budget(uint64_t total) { pool_.budget = total; }
+  uint64_t budget() const { return pool_.budget; }
+
+  std::unique_ptr<SJob> make_job(const rd_job_desc &d, int leaf, std::string *e) {
+    return m[0]->make_job(d, leaf, e);
+  }
+  bool fits_alone(const SJob &j) const { return m[0]->fits_alone(j); }
+
+  void enqueue(std::unique_ptr<SJob> j) {
+    size_t best = 0;
+    uint64_t least = ~0ull;
+    for (size_t i = 0; i < m.size(); ++i) {
+      const uint64_t p = m[i]->pending_paths();
+      if (p < least) {
+        least = p;
+        best = i;
+      }
+    }
+    m[best]->enqueue(std::move(j));
+  }
+
+  bool idle() const {
+    for (const auto &s : m)
+      if (!s->idle()) return false;
+    return true;
+  }
+
+  SJob *job(uint64_t id) {
+    for (auto &s : m)
+      if (SJob *j = s->job(id)) return j;
+    return nullptr;
+  }
+  void forget(uint64_t id) {
+    for (auto &s : m) s->forget(id);
+  }
+
+  bool ensure_ready() {
+    for (auto &s : m)
+      if (!s->ensure_ready()) {
+        err = s->err;
+        return false;
+      }
+    return true;
+  }
+
+  bool pump() {
+    ++pumps;
+    for (size_t i = 0; i < m.size(); ++i) {
+      Streamer &s = *m[i];
+      if (launched_[i]) {
+        launched_[i] = 0;
+        if (!s.harvest()) {
+          err = s.err;
+          return false;
+        }
+      }
+      if (!s.idle()) {
+        if (!s.launch()) {
+          err = s.err;
+          return false;
+        }
+        launched_[i] = 1;
+      }
+    }
+    return true;
+  }
+
+  uint64_t iterations() const { return sum([](const Streamer &s) { return s.iterations(); }); }
+  uint64_t graph_launches() const { return sum([](const Streamer &s) { return s.graph_launches; }); }
+  uint64_t paths_done() const { return sum([](const Streamer &s) { return s.paths_done; }); }
+  uint64_t hbm_in_use() const { return pool_.in_use; }
+  uint64_t hbm_peak() const { return pool_.peak; }
+  uint64_t engine_bytes() const { return sum([](const Streamer &s) { return s.engine_bytes(); }); }
+  uint32_t max_resident() const { return pool_.max_resident; }
+  uint32_t n_resident() const { return (uint32_t)sum([](const Streamer &s) { return s.n_resident(); }); }
+  uint32_t n_queued() const { return (uint32_t)sum([](const Streamer &s) { return s.n_queued(); }); }
+
+ private:
+  std::vector<uint8_t> launched_;  // member has iterations not yet harvested
+  HbmPool pool_;
+
+  template <class F> uint64_t sum(F f) const {
+    uint64_t n = 0;
+    for (const auto &s : m) n += (uint64_t)f(*s);
+    return n;
+  }
+  bool grow(int n) {
+    while ((int)m.size() < n) {
+      auto s = std::make_unique<Streamer>();
+      if (!m.empty()) {  // same settings as the first member
+        s->eng.opt = m[0]->eng.opt;
+        s->use_graph = m[0]->use_graph;
+        s->iters_per_pump = m[0]->iters_per_pump;
+        s->default_pass = m[0]->default_pass;
+      }
+      if (!s->init()) {
+        err = s->err;
+        return false;
+      }
+      s->pool = &pool_;
+      m.push_back(std::move(s));
+      launched_.push_back(0);
     }
     return true;
   }
@@ -503,7 +683,7 @@ class Streamer {
 
 // ------------------------------------------------------------- batch ABI ---
 struct rd_batch {
-  rdh::Streamer st;
+  rdh::StreamerGroup st;
   std::string last_error;
   rd_batch_stats stats{};
   uint64_t next_id = 1;
@@ -559,6 +739,20 @@ bool engine_option(rdh::Streamer &st, const std::string &k, int64_t v, std::stri
   return true;
 }
 
+// Options of a group: "pipes" (members) while nothing has rendered, every
+// other option on every member.
+bool group_option(rdh::StreamerGroup &g, const std::string &k, int64_t v, std::string *err) {
+  if (k == "pipes") {
+    if (v < 1 || v > 4) { *err = "pipes must be in 1..4"; return false; }
+    if ((int)v != (int)g.m.size() && !g.unused()) { *err = "pipes is fixed once rendering started"; return false; }
+    if (!g.resize((int)v)) { *err = g.err; return false; }
+    return true;
+  }
+  for (auto &s : g.m)
+    if (!engine_option(*s, k, v, err)) return false;
+  return true;
+}
+
 bool engine_option_get(const rdh::Streamer &st, const std::string &k, int64_t *v) {
   const rdh::EngineOptions &o = st.eng.opt;
   if (k == "block") *v = o.block;
@@ -605,14 +799,14 @@ bool validate_job(const rd_job_desc &d, std::string *e, rd_status *st) {
 }
 
 rd_status run_batch(rd_batch *b, const rd_job_desc *jobs, uint32_t n) {
-  rdh::Streamer &st = b->st;
+  rdh::StreamerGroup &st = b->st;
   for (uint32_t i = 0; i < n; ++i) {
     std::string e;
     rd_status s;
     if (!validate_job(jobs[i], &e, &s)) return batch_fail(b, s, "job " + std::to_string(i) + ": " + e);
   }
   const auto t0 = std::chrono::steady_clock::now();
-  const uint64_t it0 = st.iterations(), g0 = st.graph_launches, p0 = st.pumps, paths0 = st.paths_done;
+  const uint64_t it0 = st.iterations(), g0 = st.graph_launches(), p0 = st.pumps, paths0 = st.paths_done();
   // Host-side scene digestion (BVH builds) runs on a thread pool and
   // overlaps the GPU: jobs join the work list in submission order as soon as
   // they are built.
@@ -622,7 +816,7 @@ rd_status run_batch(rd_batch *b, const rd_job_desc *jobs, uint32_t n) {
   for (uint32_t i = 0; i < n; ++i) ready[i] = 0;
   std::atomic<uint32_t> next{0};
   std::atomic<int64_t> build_ns{0};
-  const int leaf = st.eng.opt.leaf;
+  const int leaf = st.first().eng.opt.leaf;
   auto worker = [&] {
     for (uint32_t i; (i = next.fetch_add(1)) < n;) {
       const auto b0 = std::chrono::steady_clock::now();
@@ -668,9 +862,9 @@ rd_status run_batch(rd_batch *b, const rd_job_desc *jobs, uint32_t n) {
   b->stats.host_threads = std::min(hw, n);
   b->stats.elapsed_ns = rdh::ns_between(t0, std::chrono::steady_clock::now());
   b->stats.iterations = st.iterations() - it0;
-  b->stats.graph_launches = st.graph_launches - g0;
+  b->stats.graph_launches = st.graph_launches() - g0;
   b->stats.pumps = st.pumps - p0;
-  b->stats.paths = st.paths_done - paths0;
+  b->stats.paths = st.paths_done() - paths0;
   return RD_OK;
 }
 
@@ -683,33 +877,39 @@ rd_status rd_batch_create(rd_batch **out) {
   *out = nullptr;
   auto *b = new (std::nothrow) rd_batch();
   if (!b) return RD_ERR_OUT_OF_MEMORY;
-  b->st.iters_per_pump = 4;  // throughput over latency: fewer host round trips
-  b->st.eng.opt.leaf = 2;    // measured fastest on mixed batches of small scenes
-  if (!b->st.init()) {
+  if (!b->st.init(2)) {  // two pipes
     rdh::set_global_error(b->st.err);
     delete b;
     return RD_ERR_DEVICE;
   }
+  b->st.each([](rdh::Streamer &s) {
+    s.iters_per_pump = 4;  // throughput over latency: fewer host round trips
+    s.eng.opt.leaf = 2;    // measured fastest on mixed batches of small scenes
+  });
   *out = b;
   return RD_OK;
 }
 
 void rd_batch_destroy(rd_batch *b) {
   if (!b) return;
-  (void)hipStreamSynchronize(b->st.eng.stream);
+  b->st.each([](rdh::Streamer &s) { (void)hipStreamSynchronize(s.eng.stream); });
   delete b;
 }
 
 rd_status rd_batch_set_option(rd_batch *b, const char *key, int64_t value) {
   if (!b || !key) return RD_ERR_INVALID_ARGUMENT;
   std::string e;
-  if (!engine_option(b->st, key, value, &e)) return batch_fail(b, RD_ERR_INVALID_ARGUMENT, e);
+  if (!group_option(b->st, key, value, &e)) return batch_fail(b, RD_ERR_INVALID_ARGUMENT, e);
   return RD_OK;
 }
 
 rd_status rd_batch_get_option(const rd_batch *b, const char *key, int64_t *value) {
   if (!b || !key || !value) return RD_ERR_INVALID_ARGUMENT;
-  return engine_option_get(b->st, key, value) ? RD_OK : RD_ERR_INVALID_ARGUMENT;
+  if (std::string(key) == "pipes") {
+    *value = (int64_t)b->st.m.size();
+    return RD_OK;
+  }
+  return engine_option_get(b->st.first(), key, value) ? RD_OK : RD_ERR_INVALID_ARGUMENT;
 }
 
 rd_status rd_batch_render(rd_batch *b, const rd_job_desc *jobs, uint32_t n) {
@@ -734,21 +934,28 @@ rd_status rd_batch_autotune(rd_batch *b, const rd_job_desc *jobs, uint32_t n, rd
   const int leaves[] = {1, 2, 4, 8};
   double best = 1e300, dflt = 0.0;
   rd_tune_result r{};
-  r.block = b->st.eng.opt.block;
-  r.leaf = b->st.eng.opt.leaf;
-  r.queue = b->st.eng.opt.queue;
+  const rdh::EngineOptions base = b->st.first().eng.opt;
+  r.block = base.block;
+  r.leaf = base.leaf;
+  r.queue = base.queue;
   for (uint32_t q : queues) {
     rd_batch *t = nullptr;
     rd_status s = rd_batch_create(&t);
     if (s != RD_OK) return batch_fail(b, s, rd_last_error(nullptr));
     std::unique_ptr<rd_batch, void (*)(rd_batch *)> guard(t, rd_batch_destroy);
-    t->st.eng.opt = b->st.eng.opt;
-    t->st.eng.opt.queue = q;
-    t->st.use_graph = b->st.use_graph;
+    if (!t->st.resize((int)b->st.m.size())) return batch_fail(b, RD_ERR_DEVICE, t->st.err);
+    const int use_graph = b->st.first().use_graph;
+    t->st.each([&](rdh::Streamer &s) {
+      s.eng.opt = base;
+      s.eng.opt.queue = q;
+      s.use_graph = use_graph;
+    });
     for (int blk : blocks)
       for (int lf : leaves) {
-        t->st.eng.opt.block = blk;
-        t->st.eng.opt.leaf = lf;
+        t->st.each([&](rdh::Streamer &s) {
+          s.eng.opt.block = blk;
+          s.eng.opt.leaf = lf;
+        });
         // one untimed run per launch shape compiles / warms the kernels
         if ((s = run_batch(t, jobs, n)) != RD_OK) return batch_fail(b, s, t->last_error);
         if ((s = run_batch(t, jobs, n)) != RD_OK) return batch_fail(b, s, t->last_error);
@@ -765,9 +972,11 @@ rd_status rd_batch_autotune(rd_batch *b, const rd_job_desc *jobs, uint32_t n, rd
   }
   r.best_ms = best;
   r.default_ms = dflt;
-  b->st.eng.opt.block = r.block;
-  b->st.eng.opt.leaf = r.leaf;
-  if (!b->st.eng.q_alloc) b->st.eng.opt.queue = r.queue;
+  b->st.each([&](rdh::Streamer &s) {
+    s.eng.opt.block = r.block;
+    s.eng.opt.leaf = r.leaf;
+    if (!s.eng.q_alloc) s.eng.opt.queue = r.queue;
+  });
   *out = r;
   return RD_OK;
 }
@@ -776,7 +985,7 @@ rd_status rd_batch_autotune(rd_batch *b, const rd_job_desc *jobs, uint32_t n, rd
 
 // ------------------------------------------------------------ server ABI ---
 struct rd_server {
-  rdh::Streamer st;
+  rdh::StreamerGroup st;
   std::string last_error;
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
@@ -836,12 +1045,12 @@ void server_loop(rd_server *s) {
         s->st.forget(kv.first);
       }
     }
-    s->stats.paths = s->st.paths_done;
+    s->stats.paths = s->st.paths_done();
     s->stats.iterations = s->st.iterations();
-    s->stats.hbm_in_use = s->st.hbm_in_use;
-    s->stats.hbm_peak = s->st.hbm_peak;
+    s->stats.hbm_in_use = s->st.hbm_in_use();
+    s->stats.hbm_peak = s->st.hbm_peak();
     s->stats.engine_bytes = s->st.engine_bytes();
-    s->stats.max_resident = s->st.max_resident;
+    s->stats.max_resident = s->st.max_resident();
     s->stats.jobs_resident = s->st.n_resident();
     s->stats.jobs_queued = s->st.n_queued() + (uint32_t)s->inbox.size();
     s->cv_done.notify_all();
@@ -868,20 +1077,22 @@ rd_status rd_server_create(const rd_server_config *cfg, rd_server **out) {
   *out = nullptr;
   auto *s = new (std::nothrow) rd_server();
   if (!s) return RD_ERR_OUT_OF_MEMORY;
-  if (!s->st.init()) {
+  if (!s->st.init(2)) {  // two pipes
     rdh::set_global_error(s->st.err);
     delete s;
     return RD_ERR_DEVICE;
   }
-  s->st.default_pass = 8;
-  s->st.eng.opt.leaf = 2;
+  s->st.each([](rdh::Streamer &m) {
+    m.default_pass = 8;
+    m.eng.opt.leaf = 2;
+  });
   if (cfg) {
-    if (cfg->hbm_budget_bytes) s->st.budget = cfg->hbm_budget_bytes;
-    if (cfg->pass_samples) s->st.default_pass = cfg->pass_samples;
+    if (cfg->hbm_budget_bytes) s->st.set_budget(cfg->hbm_budget_bytes);
+    if (cfg->pass_samples) s->st.each([&](rdh::Streamer &m) { m.default_pass = cfg->pass_samples; });
     std::string e;
-    if ((cfg->block && !engine_option(s->st, "block", cfg->block, &e)) ||
-        (cfg->queue && !engine_option(s->st, "queue", cfg->queue, &e)) ||
-        (cfg->leaf && !engine_option(s->st, "leaf", cfg->leaf, &e))) {
+    if ((cfg->block && !group_option(s->st, "block", cfg->block, &e)) ||
+        (cfg->queue && !group_option(s->st, "queue", cfg->queue, &e)) ||
+        (cfg->leaf && !group_option(s->st, "leaf", cfg->leaf, &e))) {
       rdh::set_global_error(e);
       delete s;
       return RD_ERR_INVALID_ARGUMENT;
@@ -892,7 +1103,7 @@ rd_status rd_server_create(const rd_server_config *cfg, rd_server **out) {
     delete s;
     return RD_ERR_DEVICE;
   }
-  if (s->st.engine_bytes() > s->st.budget) {
+  if (s->st.engine_bytes() > s->st.budget()) {
     rdh::set_global_error("HBM budget smaller than the engine's own buffers");
     delete s;
     return RD_ERR_BUDGET;
@@ -922,7 +1133,7 @@ rd_status rd_server_submit(rd_server *s, const rd_job_desc *job, uint64_t *job_i
     std::lock_guard<std::mutex> lk(s->mu);
     if (s->failed) return RD_ERR_DEVICE;
     if (s->stop) return RD_ERR_INVALID_ARGUMENT;
-    leaf = s->st.eng.opt.leaf;
+    leaf = s->st.first().eng.opt.leaf;
   }
   auto j = s->st.make_job(*job, leaf, &e);  // BVH build on the caller's thread
   std::lock_guard<std::mutex> lk(s->mu);
