@@ -56,7 +56,7 @@ void rd_batch_destroy(rd_batch *b);
  * "grid" (workgroups, 0 = auto), "refill" (0..64, as rd_set_option),
  * "graph" (1 = replay a captured hipGraph),
  * "l_capacity" (radiance slots), "iters_per_pump" (even, >= 2),
- * "pipes" (1..4, default 2; settable until the first render): independent
+ * "pipes" (1..4, default 3; settable until the first render): independent
  * engines, each with its own stream, queues and radiance ring; every job is
  * given whole to the one with the fewest pending paths and the host pumps
  * them in turn, so their kernels overlap on the GPU. */
