@@ -56,6 +56,9 @@ void rd_batch_destroy(rd_batch *b);
  * "grid" (workgroups, 0 = auto), "refill" (0..64, as rd_set_option),
  * "graph" (1 = replay a captured hipGraph),
  * "l_capacity" (radiance slots), "iters_per_pump" (even, >= 2),
+ * "ticket_cap" (passes in the work list, default 2^20, fixed once rendering
+ * started: the list starts over whenever the engine has drained, and at the
+ * cap new passes wait for that), read-only "rewinds" (first pipe's count),
  * "pipes" (1..4, default 3; settable until the first render): independent
  * engines, each with its own stream, queues and radiance ring; every job is
  * given whole to the one with the fewest pending paths and the host pumps

@@ -65,6 +65,12 @@ rd_status rd_debug_host_bvh_info(const rd_scene *scene, int leaf_size, uint32_t 
 rd_status rd_debug_host_trace(const rd_scene *scene, int leaf_size, const float *rays,
                               uint32_t n, int32_t *ids, float *ts);
 
+/* Host: the streaming engine's radiance-ring allocator.  ops[i] >= 0 reserves
+ * that many slots for the next ticket (tickets are numbered 0, 1, ... in
+ * reservation order, refused ones included) and writes the offset, or -1 if
+ * it does not fit now; ops[i] < 0 releases ticket -ops[i]-1 and writes 0. */
+rd_status rd_debug_ring_ops(uint64_t capacity, const int64_t *ops, uint32_t n, int64_t *out);
+
 /* Host: the same tree collapsed to 4-wide nodes.  bvh4_info reports its node
  * count and the stack depth its traversal needs (the kernel keeps
  * stack4 + 3 rows) and checks that every primitive sits in exactly one leaf;

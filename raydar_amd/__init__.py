@@ -74,6 +74,7 @@ def lib():
                                              C.c_char_p, C.c_size_t]),
         "rd_debug_host_trace": (C.c_int, [sp, C.c_int, C.c_void_p, C.c_uint32, C.c_void_p,
                                           C.c_void_p]),
+        "rd_debug_ring_ops": (C.c_int, [C.c_uint64, C.c_void_p, C.c_uint32, C.c_void_p]),
         "rd_debug_host_bvh4_info": (C.c_int, [sp, C.c_int, P(C.c_uint32), P(C.c_uint32),
                                               C.c_char_p, C.c_size_t]),
         "rd_debug_host_trace4": (C.c_int, [sp, C.c_int, C.c_void_p, C.c_uint32, C.c_void_p,
@@ -304,6 +305,14 @@ def host_bvh_info(scene, leaf=4):
     s = scene.as_rd()
     st = lib().rd_debug_host_bvh_info(C.byref(s), leaf, C.byref(n), C.byref(d), err, 256)
     return st, n.value, d.value, err.value.decode()
+
+
+def ring_ops(capacity, ops):
+    """Replays reserve (>= 0) / release (< 0) operations on the radiance ring."""
+    ops = np.ascontiguousarray(ops, np.int64)
+    out = np.zeros(len(ops), np.int64)
+    check(lib().rd_debug_ring_ops(capacity, _ptr(ops), len(ops), _ptr(out)))
+    return out
 
 
 def host_bvh4_info(scene, leaf=4):

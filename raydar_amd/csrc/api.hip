@@ -20,6 +20,7 @@
 #include "../../include/raydar_hip_debug.h"
 #include "bvh.hpp"
 #include "engine.hpp"
+#include "ring.hpp"
 #include "kernels.hpp"
 #include "pathmath.hpp"
 
@@ -745,6 +746,23 @@ rd_status rd_debug_host_trace(const rd_scene *scene, int leaf, const float *rays
     return RD_ERR_INVALID_ARGUMENT;
   }
   for (uint32_t i = 0; i < n; ++i) ids[i] = rdh::host_trace(b, rays + 6ull * i, rays + 6ull * i + 3, ts + i);
+  return RD_OK;
+}
+
+rd_status rd_debug_ring_ops(uint64_t capacity, const int64_t *ops, uint32_t n, int64_t *out) {
+  if ((n && (!ops || !out)) || capacity == 0) return RD_ERR_INVALID_ARGUMENT;
+  rdh::RadianceRing ring(capacity);
+  uint32_t tickets = 0;
+  for (uint32_t i = 0; i < n; ++i) {
+    if (ops[i] >= 0) {
+      uint64_t off = 0;
+      out[i] = ring.alloc((uint64_t)ops[i], tickets, &off) ? (int64_t)off : -1;
+      ++tickets;
+    } else {
+      ring.release((uint32_t)(-ops[i] - 1));
+      out[i] = 0;
+    }
+  }
   return RD_OK;
 }
 

@@ -31,6 +31,7 @@
 #include "bvh.hpp"
 #include "engine.hpp"
 #include "kernels.hpp"
+#include "ring.hpp"
 
 namespace rdh {
 
@@ -39,7 +40,7 @@ namespace {
 using Clock = std::chrono::steady_clock;
 
 constexpr uint32_t kJobSlots = 4096;        // device job table entries
-constexpr uint32_t kTicketCap = 1u << 20;   // device ticket table entries
+constexpr uint32_t kTicketCap = 1u << 20;   // default device ticket table entries
 
 int64_t ns_between(Clock::time_point a, Clock::time_point b) {
   return std::chrono::duration_cast<std::chrono::nanoseconds>(b - a).count();
@@ -76,12 +77,6 @@ struct HbmPool {
   uint32_t resident = 0, max_resident = 0;
 };
 
-struct LAlloc {
-  uint64_t off, size;
-  uint32_t ticket;
-  bool freed;
-};
-
 }  // namespace
 
 class Streamer {
@@ -93,8 +88,9 @@ class Streamer {
   int use_graph = 1;
   int iters_per_pump = 2;
   uint32_t default_pass = 0;
+  uint32_t ticket_cap = kTicketCap;  // ticket table entries (fixed once ready)
   // statistics
-  uint64_t paths_done = 0, graph_launches = 0, pumps = 0;
+  uint64_t paths_done = 0, graph_launches = 0, pumps = 0, rewinds = 0;
 
   ~Streamer() { teardown(); }
 
@@ -108,27 +104,29 @@ class Streamer {
   bool ensure_ready() {
     if (ready_) return true;
     if (!eng.ensure_buffers(&err)) return false;
-    if (!eng.ensure_tickets(kTicketCap, &err)) return false;
+    if (!eng.ensure_tickets(ticket_cap, &err)) return false;
     if (!eng.check(hipMalloc(&d_jobs_, sizeof(DevJob) * kJobSlots), "hipMalloc(job table)", &err))
       return false;
-    if (!eng.check(hipMemsetAsync(eng.d_ticket_done, 0, 8ull * kTicketCap, eng.stream), "hipMemset", &err))
+    if (!eng.check(hipMemsetAsync(eng.d_ticket_done, 0, 8ull * ticket_cap, eng.stream), "hipMemset", &err))
       return false;
     if (!eng.check(launch_set_ctl(eng.ctl, 0, 0, eng.stream), "set ctl", &err)) return false;
-    if (!eng.check(hipHostMalloc(&done_host_, 8ull * kTicketCap, hipHostMallocDefault), "hipHostMalloc", &err))
+    if (!eng.check(hipHostMalloc(&done_host_, 8ull * ticket_cap, hipHostMallocDefault), "hipHostMalloc", &err))
       return false;
     for (uint32_t s = kJobSlots; s-- > 0;) free_slots_.push_back(s);
     h_jobs_.assign(kJobSlots, DevJob{});
-    tickets_.reserve(kTicketCap);  // never reallocates: async uploads read from it
+    tickets_.reserve(ticket_cap);  // never reallocates: async uploads read from it
+    ring_.reset(eng.l_alloc);
     fixed_bytes_ = engine_bytes();
     pool->fixed += fixed_bytes_;
     ready_ = true;
     return eng.check(hipStreamSynchronize(eng.stream), "init", &err);
   }
 
+  bool ready() const { return ready_; }
   uint64_t engine_bytes_if_ready() const { return ready_ ? fixed_bytes_ : 0; }
 
   uint64_t engine_bytes() const {
-    return (uint64_t)kSegments * eng.seg_cap * (2 * 56 + 8) + eng.l_alloc * 16 + kTicketCap * 40ull +
+    return (uint64_t)kSegments * eng.seg_cap * (2 * 56 + 8) + eng.l_alloc * 16 + ticket_cap * 40ull +
            sizeof(DevJob) * (uint64_t)kJobSlots;
   }
 
@@ -197,12 +195,32 @@ class Streamer {
     // Launch-shape options may have changed since the queues were sized;
     // resize only while no path is in flight.
     if (last_count_ == 0 && ctl_total_ == last_cursor_ && !eng.ensure_buffers(&err)) return false;
+    if (!rewind()) return false;
     if (!admit()) return false;
     if (!issue()) return false;
     const bool work = ctl_total_ > last_cursor_ || last_count_ > 0;
     if (work) {
       if (!run_iterations()) return false;
     }
+    return true;
+  }
+
+  // Once every issued ticket is accumulated and no path is in flight, the
+  // ticket list starts over at index 0, so the table never fills for good
+  // (at the cap, issue() stops until the work in flight drains).
+  bool rewind() {
+    if (uploaded_ == 0 || ticket_lo_ != uploaded_ || last_count_ != 0 || last_cursor_ != ctl_total_)
+      return true;
+    if (!eng.check(hipMemsetAsync(eng.d_ticket_done, 0, 8ull * uploaded_, eng.stream), "hipMemset", &err))
+      return false;
+    if (!eng.check(launch_set_ctl(eng.ctl, 0, 0, eng.stream), "set ctl", &err)) return false;
+    tickets_.clear();
+    ticket_job_.clear();
+    ticket_fin_.clear();
+    uploaded_ = ticket_lo_ = 0;
+    ctl_total_ = last_cursor_ = 0;
+    stalls_ = 0;
+    ++rewinds;
     return true;
   }
 
@@ -234,7 +252,7 @@ class Streamer {
   uint32_t last_count_ = 0;
   uint32_t stalls_ = 0;
   int parity_ = 0;
-  std::deque<LAlloc> ring_;
+  RadianceRing ring_;
   uint64_t fixed_bytes_ = 0;
   hipGraphExec_t graph_ = nullptr;
   hipGraph_t graph_def_ = nullptr;
@@ -295,25 +313,6 @@ class Streamer {
     return true;
   }
 
-  // Ring allocation of `n` radiance slots; false when the ring is full.
-  bool ring_alloc(uint64_t n, uint64_t *off) {
-    const uint64_t cap = eng.l_alloc;
-    if (n > cap) return false;
-    if (ring_.empty()) {
-      *off = 0;
-      return true;
-    }
-    const uint64_t tail = ring_.front().off;
-    const uint64_t head = ring_.back().off + ring_.back().size;
-    if (head >= tail) {  // free space [head, cap) and [0, tail)
-      if (cap - head >= n) { *off = head; return true; }
-      if (tail >= n) { *off = 0; return true; }
-      return false;
-    }
-    if (tail - head >= n) { *off = head; return true; }
-    return false;
-  }
-
   bool issue() {
     // Keep enough work ahead of the cursor that no iteration of the next
     // pump finds the work list empty (each can admit a queue load).
@@ -321,12 +320,13 @@ class Streamer {
     bool added = false;
     for (SJob *j : resident_) {
       while (j->passes_issued < j->passes_total && ctl_total_ - last_cursor_ < ahead) {
-        if (tickets_.size() >= kTicketCap) break;
+        if (tickets_.size() >= ticket_cap) break;  // resumes after the next rewind
         const uint64_t npix = (uint64_t)j->W * j->H;
         const uint32_t s0 = j->passes_issued * j->pass_spp;
         const uint32_t spp = std::min(j->pass_spp, j->spp - s0);
+        const uint32_t ti = (uint32_t)tickets_.size();
         uint64_t off;
-        if (!ring_alloc(spp * npix, &off)) break;
+        if (!ring_.alloc(spp * npix, ti, &off)) break;
         DevTicket t{};
         t.work_begin = ctl_total_;
         t.l_offset = off;
@@ -334,11 +334,9 @@ class Streamer {
         t.sample_base = s0;
         t.spp = spp;
         t.paths = (uint32_t)(spp * npix);
-        const uint32_t ti = (uint32_t)tickets_.size();
         tickets_.push_back(t);
         ticket_job_.push_back(j);
         ticket_fin_.push_back(0);
-        ring_.push_back(LAlloc{off, (uint64_t)t.paths, ti, false});
         j->tickets.push_back(ti);
         j->passes_issued++;
         ctl_total_ += t.paths;
@@ -479,8 +477,7 @@ class Streamer {
                        "accumulate", &err))
           return false;
         ticket_fin_[ti] = 2;
-        for (LAlloc &a : ring_)
-          if (a.ticket == ti) a.freed = true;
+        ring_.release(ti);
         j->passes_done++;
         paths_done += t.paths;
         progressed = true;
@@ -510,7 +507,6 @@ class Streamer {
         finished.push_back(j);
       }
     }
-    while (!ring_.empty() && ring_.front().freed) ring_.pop_front();
     while (ticket_lo_ < n && ticket_fin_[ticket_lo_] == 2) ++ticket_lo_;
     if (!finished.empty()) {
       if (!eng.check(hipStreamSynchronize(eng.stream), "read results", &err)) return false;
@@ -666,6 +662,7 @@ class StreamerGroup {
         s->use_graph = m[0]->use_graph;
         s->iters_per_pump = m[0]->iters_per_pump;
         s->default_pass = m[0]->default_pass;
+        s->ticket_cap = m[0]->ticket_cap;
       }
       if (!s->init()) {
         err = s->err;
@@ -713,6 +710,10 @@ bool engine_option(rdh::Streamer &st, const std::string &k, int64_t v, std::stri
     o.grid = (int)v;
   } else if (k == "graph") {
     st.use_graph = v != 0;
+  } else if (k == "ticket_cap") {
+    if (v < 1 || v > (1 << 24)) { *err = "ticket_cap must be in 1..2^24"; return false; }
+    if (st.ready() && (uint32_t)v != st.ticket_cap) { *err = "ticket_cap is fixed once rendering started"; return false; }
+    st.ticket_cap = (uint32_t)v;
   } else if (k == "iters_per_pump") {
     if (v < 2 || v > 1024) { *err = "iters_per_pump out of range"; return false; }
     st.iters_per_pump = (int)v;
@@ -763,6 +764,8 @@ bool engine_option_get(const rdh::Streamer &st, const std::string &k, int64_t *v
   else if (k == "shade_blocks") *v = o.shade_blocks;
   else if (k == "graph") *v = st.use_graph;
   else if (k == "iters_per_pump") *v = st.iters_per_pump;
+  else if (k == "ticket_cap") *v = st.ticket_cap;
+  else if (k == "rewinds") *v = (int64_t)st.rewinds;
   else if (k == "l_capacity") *v = (int64_t)o.l_capacity;
   else if (k == "num_cus") *v = st.eng.num_cus;
   else return false;

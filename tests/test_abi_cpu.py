@@ -204,3 +204,47 @@ def test_nonfinite_rays_use_the_scan():
     ids_o, ts_o, _, _ = oracle.trace(sc, rays)
     ids_h, ts_h = ra.host_trace(sc, rays, 4)
     np.testing.assert_array_equal(ids_h, ids_o)
+
+
+def test_radiance_ring_wrap_to_tail_is_full():
+    # [0,40) and [40,80) of 100; release the first; [0,40) again ends exactly
+    # at the oldest live range, so the ring is full: nothing more fits
+    out = ra.ring_ops(100, [40, 40, -1, 40, 10, 20])
+    assert list(out[:2]) == [0, 40] and out[3] == 0
+    assert out[4] == -1 and out[5] == -1
+    # releasing the range at 40 frees everything behind the newest one
+    out = ra.ring_ops(100, [40, 40, -1, 40, -2, 60, 1])
+    assert out[5] == 40 and out[6] == -1
+
+
+def test_radiance_ring_never_overlaps_live_ranges():
+    rng = np.random.default_rng(5)
+    for cap in (7, 64, 1000):
+        ops, sizes, live = [], [], set()
+        for _ in range(3000):
+            if live and rng.random() < 0.45:
+                t = int(rng.choice(sorted(live)))
+                live.discard(t)
+                ops.append(-t - 1)
+            else:
+                n = int(rng.integers(1, cap // 2 + 2))
+                ops.append(n)
+                sizes.append(n)
+                live.add(len(sizes) - 1)  # refused ones are dropped below
+        out = ra.ring_ops(cap, ops)
+        # replay: live ranges (reserved, not released) must never overlap,
+        # and a reservation is refused only if the ring is not empty
+        ranges, t, alive = {}, 0, set()
+        for op, o in zip(ops, out):
+            if op >= 0:
+                if o >= 0:
+                    assert 0 <= o and o + op <= cap
+                    for a, b in (ranges[x] for x in alive):
+                        assert o + op <= a or b <= o, (cap, o, op, a, b)
+                    ranges[t] = (o, o + op)
+                    alive.add(t)
+                else:
+                    assert alive or op > cap
+                t += 1
+            else:
+                alive.discard(-op - 1)

@@ -89,6 +89,22 @@ def test_batch_pipes_option():
         np.testing.assert_array_equal(job.acc.view(np.uint32), _oracle_acc(s, spp, b, seed).view(np.uint32))
 
 
+@pytest.mark.parametrize("pipes", [1, 3])
+def test_batch_ticket_list_rewinds_at_cap(pipes):
+    # a 2-entry ticket table: passes wait for the work in flight to drain,
+    # then the list starts over; results are unchanged
+    specs = _jobs()[:3]
+    jobs = [Job(s, spp, b, seed, pass_samples=1) for s, spp, b, seed in specs]
+    with BatchRenderer(pipes=pipes, ticket_cap=2) as br:
+        assert br.get_option("ticket_cap") == 2
+        br.render(jobs)
+        assert br.get_option("rewinds") > 0
+        with pytest.raises(ra.RaydarError):
+            br.set_option("ticket_cap", 4)
+    for job, (s, spp, b, seed) in zip(jobs, specs):
+        np.testing.assert_array_equal(job.acc.view(np.uint32), _oracle_acc(s, spp, b, seed).view(np.uint32))
+
+
 def test_server_alone_equals_mixed():
     specs = _jobs()
     target = specs[1]
