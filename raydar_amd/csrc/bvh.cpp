@@ -15,6 +15,7 @@
 #include "bvh.hpp"
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
 #include <limits>
@@ -226,6 +227,15 @@ struct Builder {
 // by that child's two children, up to four.  Children follow their parent.
 // need = stack entries a traversal below the node can hold (an entered node
 // defers all but one of its k entered inner children).
+// Collapses the binary tree into 4-wide nodes, choosing which binary nodes
+// to keep by dynamic programming so that the summed surface area of the
+// 4-wide nodes (each visit tests four child boxes, and a node is visited
+// with probability proportional to its area) is minimal.  F(x, k) is the
+// least cost of representing binary subtree x in at most k slots of its
+// parent: either x stays a node (one slot, its area plus its two children
+// spread over four slots) or x dissolves and its children share the k slots.
+// Leaves take one slot and cost nothing here (they are tested whatever the
+// collapse).  The traversal's stack bound is computed on the result.
 struct Collapser {
   const std::vector<DevNode> &n2;
   std::vector<DevNode4> out;
@@ -234,6 +244,13 @@ struct Collapser {
     int32_t code;
     float box[6];  // lo x, hi x, lo y, hi y, lo z, hi z
   };
+  std::vector<std::array<double, 5>> F;   // F[x][k], k = 1..4
+  std::vector<std::array<int8_t, 5>> how;  // 0: x stays a node; j: j slots to child 0
+  std::vector<int8_t> split4;              // x as a node: slots for child 0
+
+  explicit Collapser(const std::vector<DevNode> &nodes)
+      : n2(nodes), F(nodes.size()), how(nodes.size()), split4(nodes.size(), 1) {}
+
   static double area(const Child &c) {
     const double ex = std::max(0.0, (double)c.box[1] - c.box[0]);
     const double ey = std::max(0.0, (double)c.box[3] - c.box[2]);
@@ -244,22 +261,56 @@ struct Collapser {
     const DevNode &m = n2[node];
     return Child{m.child[k], {m.lo_x[k], m.hi_x[k], m.lo_y[k], m.hi_y[k], m.lo_z[k], m.hi_z[k]}};
   }
-  int32_t collapse(int32_t b, uint32_t level, uint32_t *need) {
-    Child ch[4] = {child_of(b, 0), child_of(b, 1)};
-    int cnt = 2;
-    while (cnt < 4) {
-      int best = -1;
-      double best_area = -1.0;
-      for (int i = 0; i < cnt; ++i)
-        if (!is_leaf(ch[i].code) && area(ch[i]) > best_area) {
-          best = i;
-          best_area = area(ch[i]);
-        }
-      if (best < 0) break;
-      const int32_t m = ch[best].code;
-      ch[best] = child_of(m, 0);
-      ch[cnt++] = child_of(m, 1);
+  double cost(int32_t c, int k) const { return is_leaf(c) ? 0.0 : F[c][k]; }
+
+  // Fills F / how / split4 for the subtree of binary node b (area a).
+  void solve(int32_t b, double a) {
+    const Child c0 = child_of(b, 0), c1 = child_of(b, 1);
+    if (!is_leaf(c0.code)) solve(c0.code, area(c0));
+    if (!is_leaf(c1.code)) solve(c1.code, area(c1));
+    double best = std::numeric_limits<double>::infinity();
+    for (int j = 1; j <= 3; ++j) {
+      const double c = cost(c0.code, j) + cost(c1.code, 4 - j);
+      if (c < best) {
+        best = c;
+        split4[b] = (int8_t)j;
+      }
     }
+    const double as_node = a + best;
+    F[b][1] = as_node;
+    how[b][1] = 0;
+    for (int k = 2; k <= 4; ++k) {
+      double bk = as_node;
+      int8_t h = 0;
+      for (int j = 1; j < k; ++j) {
+        const double c = cost(c0.code, j) + cost(c1.code, k - j);
+        if (c < bk) {
+          bk = c;
+          h = (int8_t)j;
+        }
+      }
+      F[b][k] = bk;
+      how[b][k] = h;
+    }
+  }
+
+  // The slots item c takes when given at most k of them.
+  void emit(const Child &c, int k, std::vector<Child> *slots) const {
+    if (is_leaf(c.code) || how[c.code][k] == 0) {
+      slots->push_back(c);
+      return;
+    }
+    const int j = how[c.code][k];
+    emit(child_of(c.code, 0), j, slots);
+    emit(child_of(c.code, 1), k - j, slots);
+  }
+
+  int32_t collapse(int32_t b, uint32_t level, uint32_t *need) {
+    std::vector<Child> ch;
+    ch.reserve(4);
+    emit(child_of(b, 0), split4[b], &ch);
+    emit(child_of(b, 1), 4 - split4[b], &ch);
+    const int cnt = (int)ch.size();
     const uint32_t idx = (uint32_t)out.size();
     out.emplace_back();
     depth = std::max(depth, level);
@@ -394,8 +445,9 @@ bool build_scene(const rd_scene &scene, int leaf_size, BuiltScene *out, std::str
   out->depth4 = 0;
   out->stack4 = 0;
   if (!out->nodes.empty()) {
-    Collapser c{out->nodes, {}, 0};
+    Collapser c(out->nodes);
     c.out.reserve(out->nodes.size());
+    c.solve(0, 0.0);
     uint32_t need = 0;
     c.collapse(0, 1, &need);
     out->nodes4 = std::move(c.out);
