@@ -357,12 +357,24 @@ class Streamer {
 
   bool wide_ = false;
 
+  // The 4-wide tree while every resident job's walk fits the kernel's stack
+  // (both trees of every job stay resident, so this may change between any
+  // two iterations).
+  bool want_w4() const {
+    if (!eng.opt.bvh4 || wide_ || eng.opt.brute_force || eng.opt.refill <= 0) return false;
+    for (const SJob *j : resident_)
+      if (j->dev.n_nodes > 0 && (j->dev.n_nodes4 == 0 || j->dev.stack4 + 3 > (uint32_t)kStackDepth))
+        return false;
+    return true;
+  }
+
   IterArgs args() const {
     IterArgs a = eng.iter_args(d_jobs_);
     a.single_job = 0;
     a.job_cap = kJobSlots;
     a.track_done = 1;
     a.wide_stack = wide_ ? 1 : 0;
+    a.bvh4 = want_w4() ? 1 : 0;
     return a;
   }
 
@@ -379,7 +391,7 @@ class Streamer {
     hipError_t e = hipSuccess;
     for (int p = 0; p < 2 && e == hipSuccess; ++p) {
       e = launch_generate(a, p, eng.shape(0), false, eng.stream);
-      if (e == hipSuccess) e = launch_extend(a, p, eng.shape(1), false, eng.stream);
+      if (e == hipSuccess) e = launch_extend(a, p, eng.shape(1, &a), false, eng.stream);
       if (e == hipSuccess) e = launch_shade(a, p, eng.shape(2), false, eng.stream);
     }
     hipError_t e2 = hipStreamEndCapture(eng.stream, &graph_def_);
@@ -392,10 +404,11 @@ class Streamer {
     graph_refill_ = eng.opt.refill;
     graph_shade_ = eng.opt.shade_blocks;
     graph_wide_ = wide_;
+    graph_w4_ = a.bvh4 != 0;
     graph_gen_ = eng.generation;
     return true;
   }
-  bool graph_wide_ = false;
+  bool graph_wide_ = false, graph_w4_ = false;
   uint64_t graph_gen_ = 0;
 
   bool run_iterations() {
@@ -411,7 +424,7 @@ class Streamer {
     }
     if (!graph_ || graph_block_ != eng.opt.block || graph_grid_ != eng.opt.grid ||
         graph_brute_ != eng.opt.brute_force || graph_refill_ != eng.opt.refill || graph_shade_ != eng.opt.shade_blocks ||
-        graph_wide_ != wide_ ||
+        graph_wide_ != wide_ || graph_w4_ != want_w4() ||
         graph_gen_ != eng.generation) {
       if (!capture()) return false;
     }
@@ -729,6 +742,9 @@ bool engine_option(rdh::Streamer &st, const std::string &k, int64_t v, std::stri
   } else if (k == "shade_blocks") {
     if (v < 0 || v > 64) { *err = "shade_blocks must be in 0..64"; return false; }
     o.shade_blocks = (int)v;
+  } else if (k == "bvh4") {
+    if (v != 0 && v != 1) { *err = "bvh4 must be 0 or 1"; return false; }
+    o.bvh4 = (int)v;
   } else if (k == "count") {
     o.count = v != 0;
   } else if (k == "kernel_timing") {
@@ -762,6 +778,7 @@ bool engine_option_get(const rdh::Streamer &st, const std::string &k, int64_t *v
   else if (k == "grid") *v = o.grid;
   else if (k == "refill") *v = o.refill;
   else if (k == "shade_blocks") *v = o.shade_blocks;
+  else if (k == "bvh4") *v = o.bvh4;
   else if (k == "graph") *v = st.use_graph;
   else if (k == "iters_per_pump") *v = st.iters_per_pump;
   else if (k == "ticket_cap") *v = st.ticket_cap;

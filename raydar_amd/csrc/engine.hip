@@ -217,7 +217,8 @@ LaunchShape Engine::shape(int which, const IterArgs *a) const {
     const bool lanes = which == 1 && opt.refill > 0;
     const uint32_t dyn = (which == 1 && a) ? extend_dynamic_lds(*a, s.block) : 0;
     const bool w4 = which == 1 && a && a->bvh4;
-    int per_cu = std::max(1, occupancy_blocks(which, s.block, lanes, dyn ? a->lds_bvh_bytes : 0, dyn, w4));
+    const bool single = a ? a->single_job != 0 : true;
+    int per_cu = std::max(1, occupancy_blocks(which, s.block, lanes, dyn ? a->lds_bvh_bytes : 0, dyn, w4, single));
     if (which == 2 && opt.shade_blocks > 0) per_cu = std::min(per_cu, opt.shade_blocks);
     s.grid = num_cus * per_cu;
   }

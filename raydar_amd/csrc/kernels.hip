@@ -380,8 +380,13 @@ __device__ __forceinline__ void order_pair(float &ka, int32_t &ca, float &kb, in
 // five comparators, writes the three farther candidates above the stack top
 // (kept as many as were entered, minus one) and reads the top entry; the
 // stack needs DevJob::stack4 + 3 rows.
+// Waves per SIMD the lanes kernel is compiled for: the multi-job 4-wide walk
+// keeps each lane's job pointers in registers and needs 7 (72 VGPRs) to
+// avoid spilling; every other variant fits 8.
+constexpr int lanes_min_waves(bool single, bool w4) { return !single && w4 ? 7 : RDH_EXTEND_WAVES; }
+
 template <int BLOCK, bool COUNT, bool SINGLE, class StackT, bool LDS, bool W4>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(RDH_EXTEND_WAVES, 8)))
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(lanes_min_waves(SINGLE, W4), 8)))
 void k_extend_lanes(IterArgs a, int p) {
   __shared__ StackT static_stack[LDS ? 1 : kStackDepth * BLOCK];
   extern __shared__ __align__(16) unsigned char dynamic_lds[];
@@ -456,7 +461,7 @@ void k_extend_lanes(IterArgs a, int p) {
         const uint32_t n_prims = SINGLE ? j0_prims : J.n_prims;
         const uint32_t n_nodes = SINGLE ? j0_nodes : J.n_nodes;
 #ifdef RDH_CHECKED
-        cur_nodes = W4 ? j0_nodes4 : n_nodes;
+        cur_nodes = W4 ? (SINGLE ? j0_nodes4 : J.n_nodes4) : n_nodes;
         cur_prims = n_prims;
 #endif
         r = ray_pre(o, d);
@@ -482,6 +487,7 @@ void k_extend_lanes(IterArgs a, int p) {
         } else {
           if (!SINGLE) {
             nodes = J.nodes;
+            nodes4 = J.nodes4;
             prims = J.prims;
           }
           s = slab_ray(r, ray_tolerance(r, SINGLE ? j0_r0 : J.r0_tol));
@@ -963,9 +969,10 @@ template <bool COUNT, bool SINGLE>
 static const void *extend_fn(int block, bool wide, bool lanes, bool lds, bool w4) {
 #define RDH_EXT(B)                                                                           \
   if (block == B) {                                                                          \
-    if (lanes && SINGLE && !wide && w4)                                                      \
-      return lds ? (const void *)k_extend_lanes<B, COUNT, true, uint16_t, true, true>        \
-                 : (const void *)k_extend_lanes<B, COUNT, true, uint16_t, false, true>;      \
+    if (lanes && !wide && w4) {                                                              \
+      if (lds && SINGLE) return (const void *)k_extend_lanes<B, COUNT, true, uint16_t, true, true>; \
+      return (const void *)k_extend_lanes<B, COUNT, SINGLE, uint16_t, false, true>;          \
+    }                                                                                        \
     if (lanes && lds && SINGLE && !wide)                                                     \
       return (const void *)k_extend_lanes<B, COUNT, true, uint16_t, true, false>;            \
     if (lanes)                                                                               \
@@ -1005,8 +1012,8 @@ static const void *kernel_fn(int which, int block, bool count, bool single, bool
 }
 
 int occupancy_blocks(int which, int block, bool lanes, uint32_t lds_bvh_bytes,
-                     uint32_t dynamic_lds, bool w4) {
-  const void *fn = kernel_fn(which, block, false, true, false, lanes, lds_bvh_bytes > 0, w4);
+                     uint32_t dynamic_lds, bool w4, bool single) {
+  const void *fn = kernel_fn(which, block, false, single, false, lanes, lds_bvh_bytes > 0, w4);
   int n = 0;
   if (!fn || hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, block, dynamic_lds) != hipSuccess) {
     (void)hipGetLastError();

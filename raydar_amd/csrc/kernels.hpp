@@ -71,7 +71,7 @@ hipError_t launch_tonemap(const DevJob *job, uint32_t n_pixels, float spp, uchar
                           DevCounters *counters, bool count, int grid, hipStream_t st);
 // Workgroups of kernel `which` (0 generate, 1 extend, 2 shade) that fit on one CU.
 int occupancy_blocks(int which, int block, bool lanes = false, uint32_t lds_bvh_bytes = 0,
-                     uint32_t dynamic_lds = 0, bool w4 = false);
+                     uint32_t dynamic_lds = 0, bool w4 = false, bool single = true);
 
 hipError_t launch_set_ctl(QueueCtl *ctl, unsigned long long total, uint32_t n_tickets,
                           hipStream_t st);

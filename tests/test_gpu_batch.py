@@ -55,7 +55,7 @@ def test_batch_matches_oracle_and_plain():
 @pytest.mark.parametrize("opts", [dict(graph=0), dict(graph=1, block=128, grid=97),
                                   dict(block=64, leaf=1), dict(queue=4096), dict(refill=0),
                                   dict(refill=1, queue=4096), dict(pipes=1), dict(pipes=2, queue=4096),
-                                  dict(pipes=4, graph=0)])
+                                  dict(pipes=4, graph=0), dict(bvh4=0), dict(bvh4=0, graph=0, block=128)])
 def test_batch_launch_shapes_and_passes_agree(opts):
     specs = _jobs()[:3]
     a = [Job(s, spp, b, seed) for s, spp, b, seed in specs]
