@@ -902,10 +902,8 @@ rd_status rd_batch_create(rd_batch **out) {
     delete b;
     return RD_ERR_DEVICE;
   }
-  b->st.each([](rdh::Streamer &s) {
-    s.iters_per_pump = 4;  // throughput over latency: fewer host round trips
-    s.eng.opt.leaf = 2;    // measured fastest on mixed batches of small scenes
-  });
+  // throughput over latency: fewer host round trips
+  b->st.each([](rdh::Streamer &s) { s.iters_per_pump = 4; });
   *out = b;
   return RD_OK;
 }
@@ -1102,10 +1100,7 @@ rd_status rd_server_create(const rd_server_config *cfg, rd_server **out) {
     delete s;
     return RD_ERR_DEVICE;
   }
-  s->st.each([](rdh::Streamer &m) {
-    m.default_pass = 8;
-    m.eng.opt.leaf = 2;
-  });
+  s->st.each([](rdh::Streamer &m) { m.default_pass = 8; });
   if (cfg) {
     if (cfg->hbm_budget_bytes) s->st.set_budget(cfg->hbm_budget_bytes);
     if (cfg->pass_samples) s->st.each([&](rdh::Streamer &m) { m.default_pass = cfg->pass_samples; });
