@@ -225,16 +225,36 @@ LaunchShape Engine::shape(int which, const IterArgs *a) const {
   return s;
 }
 
+// Grid for an iteration known to hold at most `bound` paths: one lane per
+// path for extend and shade (never more than the occupancy grid).  A smaller
+// shade grid changes how many workgroups share a queue segment and how many
+// tiles each handles, so it is used only if the segment capacity still
+// covers the worst case, ceil(G / kSegments) * ceil(bound / (G * B)) * B.
+LaunchShape Engine::bounded_shape(int which, const IterArgs &a, uint32_t bound) const {
+  LaunchShape s = shape(which, &a);
+  if (bound == kNoBound || opt.grid > 0 || which == 0) return s;
+  const uint64_t b = (uint64_t)s.block;
+  const int g = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)s.grid, (bound + b - 1) / b));
+  if (which == 2) {
+    const uint64_t tiles = ((uint64_t)bound + (uint64_t)g * b - 1) / ((uint64_t)g * b);
+    const uint64_t need = (uint64_t)((g + kSegments - 1) / kSegments) * tiles * b;
+    if (need > seg_cap) return s;
+  }
+  s.grid = g;
+  return s;
+}
+
 bool Engine::timed(int which, std::string *err,
                    hipError_t (*fn)(const IterArgs &, int, const LaunchShape &, bool, hipStream_t),
-                   const IterArgs &a, int parity) {
+                   const IterArgs &a, int parity, uint32_t bound) {
   const bool count = opt.count != 0;
-  if (!opt.kernel_timing || count) return check(fn(a, parity, shape(which, &a), count, stream), "kernel launch", err);
+  const LaunchShape sh = bounded_shape(which, a, bound);
+  if (!opt.kernel_timing || count) return check(fn(a, parity, sh, count, stream), "kernel launch", err);
   hipEvent_t e0, e1;
   if (!check(hipEventCreate(&e0), "hipEventCreate", err)) return false;
   if (!check(hipEventCreate(&e1), "hipEventCreate", err)) return false;
   if (!check(hipEventRecord(e0, stream), "hipEventRecord", err)) return false;
-  if (!check(fn(a, parity, shape(which, &a), false, stream), "kernel launch", err)) return false;
+  if (!check(fn(a, parity, sh, false, stream), "kernel launch", err)) return false;
   if (!check(hipEventRecord(e1, stream), "hipEventRecord", err)) return false;
   if (!check(hipEventSynchronize(e1), "hipEventSynchronize", err)) return false;
   float ms = 0.f;
@@ -246,10 +266,11 @@ bool Engine::timed(int which, std::string *err,
   return true;
 }
 
-bool Engine::launch_iteration(const IterArgs &a, int parity, std::string *err) {
+bool Engine::launch_iteration(const IterArgs &a, int parity, std::string *err, uint32_t bound) {
   ++iterations;
-  return timed(0, err, launch_generate, a, parity) && timed(1, err, launch_extend, a, parity) &&
-         timed(2, err, launch_shade, a, parity);
+  // with a bound every path has been generated: generate would add nothing
+  return (bound != kNoBound || timed(0, err, launch_generate, a, parity, bound)) &&
+         timed(1, err, launch_extend, a, parity, bound) && timed(2, err, launch_shade, a, parity, bound);
 }
 
 bool Engine::run_tickets(const DevJob *d_jobs, std::vector<DevTicket> tickets,
@@ -319,6 +340,7 @@ bool Engine::run_tickets(const DevJob *d_jobs, std::vector<DevTicket> tickets,
       int parity;
       uint64_t it, cap_iters;
       bool done;
+      uint32_t bound;  // paths in flight at most (kNoBound until all are generated)
     };
     std::vector<Pipe> pipes;
     for (int k = 0; k < np; ++k) {
@@ -365,14 +387,14 @@ bool Engine::run_tickets(const DevJob *d_jobs, std::vector<DevTicket> tickets,
         }
       }
       const uint64_t cap_iters = (wk / e->q_alloc + 2) * (uint64_t)(max_bounces + 1) + 64;
-      pipes.push_back(Pipe{e, a, wk, 0, 0, cap_iters, false});
+      pipes.push_back(Pipe{e, a, wk, 0, 0, cap_iters, false, kNoBound});
     }
     const int group = std::max(1, opt.sync_every);
     for (;;) {
       for (int g = 0; g < group; ++g)
         for (Pipe &p : pipes) {
           if (p.done) continue;
-          if (!p.e->launch_iteration(p.a, p.parity, err)) return false;
+          if (!p.e->launch_iteration(p.a, p.parity, err, p.bound)) return false;
           p.parity ^= 1;
           ++p.it;
         }
@@ -397,7 +419,11 @@ bool Engine::run_tickets(const DevJob *d_jobs, std::vector<DevTicket> tickets,
         }
         if (queued(c, p.parity) == 0 && c.cursor[p.parity] == p.w) {
           p.done = true;
-        } else if (p.it > p.cap_iters) {
+        } else if (c.cursor[p.parity] == p.w) {
+          // every path generated: later iterations only shrink the queue
+          p.bound = queued(c, p.parity);
+        }
+        if (!p.done && p.it > p.cap_iters) {
           *err = "wavefront loop made no progress (internal error)";
           return false;
         }

@@ -104,15 +104,19 @@ class Engine {
   unsigned long long *d_ticket_done = nullptr;
 
   IterArgs iter_args(const DevJob *d_jobs) const;
+  static constexpr uint32_t kNoBound = 0xffffffffu;
   LaunchShape shape(int which, const IterArgs *a = nullptr) const;
-  bool launch_iteration(const IterArgs &a, int parity, std::string *err);
+  LaunchShape bounded_shape(int which, const IterArgs &a, uint32_t bound) const;
+  // One generate/extend/shade round.  `bound`: the host knows every path has
+  // been generated and at most `bound` are in flight (kNoBound otherwise).
+  bool launch_iteration(const IterArgs &a, int parity, std::string *err, uint32_t bound = kNoBound);
   bool ensure_tickets(uint32_t n, std::string *err);
 
  private:
   bool timed(int which, std::string *err, hipError_t (*fn)(const IterArgs &, int,
                                                              const LaunchShape &, bool,
                                                              hipStream_t),
-             const IterArgs &a, int parity);
+             const IterArgs &a, int parity, uint32_t bound);
   std::vector<hipEvent_t> ev_pool_;
   std::vector<std::unique_ptr<Engine>> aux_;  // pipes 1.. of run_tickets
 };
