@@ -206,7 +206,8 @@ rd_status rd_read_accumulator(rd_renderer *r, float *out, size_t float_capacity)
  *   "pipes"       a pass of k >= 2 samples is divided by samples over up to
  *                 min(pipes, k) independent streams whose kernels run
  *                 concurrently, each taking at least queue/2 paths
- *                 (1..4, default 2; 1 = one stream); read-only
+ *                 (1..4; 1 = one stream; 0 = automatic, the default: 2 for
+ *                 scenes traversed from LDS, else 3); read-only
  *                 "pipes_used" reports the last pass's count
  *   "shade_blocks" shade workgroups per CU (0 = as many as fit)
  *   "count"       1 = collect algorithmic event counters in the next passes */

@@ -412,7 +412,7 @@ rd_status rd_set_option(rd_renderer *r, const char *key, int64_t v) {
     if (v != 0 && v != 1) return r->fail(RD_ERR_INVALID_ARGUMENT, "bvh4 must be 0 or 1");
     o.bvh4 = (int)v;
   } else if (k == "pipes") {
-    if (v < 1 || v > 4) return r->fail(RD_ERR_INVALID_ARGUMENT, "pipes must be in 1..4");
+    if (v < 0 || v > 4) return r->fail(RD_ERR_INVALID_ARGUMENT, "pipes must be in 0..4 (0 = automatic)");
     o.pipes = (int)v;
   } else if (k == "shade_blocks") {
     if (v < 0 || v > 64) return r->fail(RD_ERR_INVALID_ARGUMENT, "shade_blocks must be in 0..64");

@@ -304,8 +304,15 @@ bool Engine::run_tickets(const DevJob *d_jobs, std::vector<DevTicket> tickets,
     std::vector<std::vector<DevTicket>> parts;
     int np = 1;
     if (traced.size() == 1 && host_jobs.size() == 1) {
+      // automatic: two pipes when the scene is traversed from LDS (extend
+      // bound by VALU issue), three otherwise (extend also waits on node
+      // fetches, so a third stream hides more of it)
+      const DevJob &j = host_jobs[0];
+      const uint64_t tree = opt.bvh4 && j.n_nodes4 ? 128ull * j.n_nodes4 : 64ull * j.n_nodes;
+      const bool lds = tree + 32ull * j.n_prims <= opt.lds_bvh;
+      const int want = opt.pipes > 0 ? opt.pipes : (lds ? 2 : 3);
       const uint64_t by_size = traced[0].paths / std::max<uint64_t>(1, q_alloc / 2);
-      np = (int)std::min<uint64_t>({(uint64_t)std::max(1, opt.pipes), traced[0].spp, by_size});
+      np = (int)std::min<uint64_t>({(uint64_t)want, traced[0].spp, by_size});
       np = std::max(np, 1);
     }
     pipes_used = np;

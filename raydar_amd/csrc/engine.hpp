@@ -48,8 +48,9 @@ struct EngineOptions {
   int count = 0;
   int kernel_timing = 0;
   int sync_every = 4;             // iterations between completion checks
-  int pipes = 2;                  // one-scene passes: independent streams the
-                                  // samples are divided over (1..4)
+  int pipes = 0;                  // one-scene passes: independent streams the
+                                  // samples are divided over (1..4; 0 = 2 for
+                                  // LDS-resident scenes, else 3)
 };
 
 class Engine {

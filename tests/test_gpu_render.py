@@ -282,16 +282,18 @@ def test_event_counts_agree_between_trees():
         assert a[k] > 0 and b[k] > 0, k
 
 
-@pytest.mark.parametrize("spp,pipes,queue,used", [(3, 2, 4096, 2), (5, 4, 4096, 4), (2, 4, 4096, 2),
-                                                   (1, 2, 4096, 1), (5, 4, 1 << 16, 2),
-                                                   (5, 2, 1 << 23, 1)])
-def test_pass_divided_over_pipes_matches_oracle(spp, pipes, queue, used):
+@pytest.mark.parametrize("spp,pipes,queue,lds,used", [
+    (3, 2, 4096, 14336, 2), (5, 4, 4096, 14336, 4), (2, 4, 4096, 14336, 2), (1, 2, 4096, 14336, 1),
+    (5, 4, 1 << 16, 14336, 2), (5, 2, 1 << 23, 14336, 1),
+    (5, 0, 4096, 14336, 2),  # automatic: the tree is traversed from LDS
+    (5, 0, 4096, 0, 3)])     # automatic: the tree stays in global memory
+def test_pass_divided_over_pipes_matches_oracle(spp, pipes, queue, lds, used):
     # a pass of spp samples runs on up to min(pipes, spp) streams, each
     # tracing a contiguous run of samples into its own radiance slots and
     # each given at least half a queue of paths (160x90 = 14400 per sample)
     sc = scenegen.plain_1080p(1, 160, 90)
     with ra.HipRenderer(max_sample_count=spp, max_bounces=12, seed=4, pipes=pipes,
-                        queue=queue) as r:
+                        queue=queue, lds_bvh=lds) as r:
         img = r.render_frame(sc)
         acc = r.accumulator()
         assert r.get_option("pipes_used") == used
@@ -299,8 +301,8 @@ def test_pass_divided_over_pipes_matches_oracle(spp, pipes, queue, used):
     np.testing.assert_array_equal(acc.view(np.uint32), ref.view(np.uint32))
     np.testing.assert_array_equal(img.reshape(-1), oracle.tonemap(ref, spp))
     with ra.HipRenderer(max_sample_count=spp, max_bounces=12, seed=4) as r:
-        assert r.get_option("pipes") == 2
-        for bad in (0, 5):
+        assert r.get_option("pipes") == 0  # automatic
+        for bad in (-1, 5):
             with pytest.raises(ra.RaydarError) as e:
                 r.set_option("pipes", bad)
             assert e.value.status == abi.RD_ERR_INVALID_ARGUMENT
