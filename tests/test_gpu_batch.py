@@ -186,3 +186,17 @@ def test_autotune_returns_a_valid_setting():
         out = [Job(scenegen.plain_small(0), 2, 4, 0)]
         br.render(out)
     np.testing.assert_array_equal(out[0].acc, _oracle_acc(scenegen.plain_small(0), 2, 4, 0))
+
+
+def test_batch_full_size_job_among_small_ones():
+    # a full 1080p job (2 spp, 4.1 M paths) sharing three pipes with small
+    # jobs; it must come out as rendered alone, equal to the oracle
+    big = (scenegen.plain_1080p(1), 2, 12, 21)
+    specs = [big] + _jobs()[:3]
+    jobs = [Job(s, spp, b, seed) for s, spp, b, seed in specs]
+    with BatchRenderer() as br:
+        br.render(jobs)
+    ref = oracle.render(*big, threads=max(1, min(16, os.cpu_count() or 1)))
+    np.testing.assert_array_equal(jobs[0].acc.view(np.uint32), ref.view(np.uint32))
+    for job, (s, spp, b, seed) in zip(jobs[1:], specs[1:]):
+        np.testing.assert_array_equal(job.acc.view(np.uint32), _oracle_acc(s, spp, b, seed).view(np.uint32))

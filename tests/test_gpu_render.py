@@ -306,3 +306,33 @@ def test_pass_divided_over_pipes_matches_oracle(spp, pipes, queue, lds, used):
             with pytest.raises(ra.RaydarError) as e:
                 r.set_option("pipes", bad)
             assert e.value.status == abi.RD_ERR_INVALID_ARGUMENT
+
+
+def _host_threads():
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+def test_full_size_plain_1080p_matches_oracle():
+    # the 1080p configuration at full resolution: 5 spp is 10.4 M paths,
+    # enough for two pipes at the default 8 M-path queue
+    sc = scenegen.plain_1080p(1)
+    with ra.HipRenderer(max_sample_count=5, max_bounces=12, seed=7) as r:
+        img = r.render_frame(sc)
+        acc = r.accumulator()
+        assert r.get_option("pipes_used") == 2
+    ref = oracle.render(sc, 5, 12, 7, threads=_host_threads())
+    np.testing.assert_array_equal(acc.view(np.uint32), ref.view(np.uint32))
+    np.testing.assert_array_equal(img.reshape(-1), oracle.tonemap(ref, 5))
+
+
+def test_full_size_clustered_1080p_matches_oracle_on_pixels():
+    # 4096 clustered primitives at full resolution, 8 spp (three pipes, tree
+    # in global memory); the oracle's linear scan checks every 257th pixel
+    sc = scenegen.clustered(2)
+    with ra.HipRenderer(max_sample_count=8, max_bounces=12, seed=9) as r:
+        r.render_frame(sc)
+        acc = r.accumulator()
+        assert r.get_option("pipes_used") == 3
+    pix = np.arange(0, sc.width * sc.height, 257, dtype=np.uint32)
+    ref = oracle.render(sc, 8, 12, 9, pixels=pix, threads=_host_threads())
+    np.testing.assert_array_equal(acc[pix].view(np.uint32), ref.view(np.uint32))
