@@ -695,6 +695,7 @@ class StreamerGroup {
 struct rd_batch {
   rdh::StreamerGroup st;
   std::string last_error;
+  bool failed = false;  // a device error left work behind: the engines are unusable
   rd_batch_stats stats{};
   uint64_t next_id = 1;
 };
@@ -820,6 +821,7 @@ bool validate_job(const rd_job_desc &d, std::string *e, rd_status *st) {
 
 rd_status run_batch(rd_batch *b, const rd_job_desc *jobs, uint32_t n) {
   rdh::StreamerGroup &st = b->st;
+  if (b->failed) return batch_fail(b, RD_ERR_DEVICE, "the batch renderer failed earlier: " + b->last_error);
   for (uint32_t i = 0; i < n; ++i) {
     std::string e;
     rd_status s;
@@ -875,6 +877,15 @@ rd_status run_batch(rd_batch *b, const rd_job_desc *jobs, uint32_t n) {
   for (auto &t : pool) t.join();
   // after a failure, drain what was already submitted so no job is left half-rendered
   while (status != RD_OK && !st.idle() && st.pump()) {
+  }
+  if (status != RD_OK) {
+    // nothing may still be writing the caller's buffers when this returns
+    st.each([](rdh::Streamer &s) { (void)hipStreamSynchronize(s.eng.stream); });
+    if (!st.idle()) {
+      // the drain itself failed: the jobs stay owned (destroy releases them)
+      b->failed = true;
+      return status;
+    }
   }
   for (uint64_t id : ids) st.forget(id);
   if (status != RD_OK) return status;
@@ -1073,6 +1084,8 @@ void server_loop(rd_server *s) {
     s->stats.jobs_queued = s->st.n_queued() + (uint32_t)s->inbox.size();
     s->cv_done.notify_all();
     if (!ok) {
+      // nothing may still be writing the callers' buffers once they see the error
+      s->st.each([](rdh::Streamer &m) { (void)hipStreamSynchronize(m.eng.stream); });
       s->failed = true;
       s->last_error = s->st.err;
       for (auto &kv : s->results)

@@ -350,6 +350,16 @@ bool Engine::run_tickets(const DevJob *d_jobs, std::vector<DevTicket> tickets,
       uint32_t bound;  // paths in flight at most (kNoBound until all are generated)
     };
     std::vector<Pipe> pipes;
+    // On an early return (error), every pipe's stream is drained first: their
+    // kernels write this engine's radiance arena, which the caller may free.
+    struct DrainOnError {
+      std::vector<Pipe> &p;
+      bool ok = false;
+      ~DrainOnError() {
+        if (!ok)
+          for (Pipe &x : p) (void)hipStreamSynchronize(x.e->stream);
+      }
+    } drain{pipes};
     for (int k = 0; k < np; ++k) {
       Engine *e = k == 0 ? this : aux_[k - 1].get();
       if (k > 0) {
@@ -438,6 +448,7 @@ bool Engine::run_tickets(const DevJob *d_jobs, std::vector<DevTicket> tickets,
       }
       if (all) break;
     }
+    drain.ok = true;
     for (int k = 1; k < np; ++k) {  // the pipes' kernel statistics are this engine's
       Engine &e = *aux_[k - 1];
       for (int i = 0; i < 5; ++i) {
