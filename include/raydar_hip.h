@@ -192,7 +192,8 @@ rd_status rd_read_accumulator(rd_renderer *r, float *out, size_t float_capacity)
 
 /* Tuning knobs, normally chosen by the autotuner.  Keys:
  *   "block"       workgroup size of the wavefront kernels: 64, 128 or 256
- *   "queue"       path-queue capacity (paths in flight), >= 1024
+ *   "queue"       path-queue capacity (paths in flight per pipe), >= 1024,
+ *                 default 2^24
  *   "leaf"        BVH leaf size 1..8, default 1 (takes effect at the next new_frame)
  *   "grid"        workgroups per persistent launch (0 = automatic)
  *   "traversal"   0 = BVH, 1 = brute-force linear scan (cpu.rs:345-349)
@@ -205,7 +206,7 @@ rd_status rd_read_accumulator(rd_renderer *r, float *out, size_t float_capacity)
  *                 0 = the binary tree
  *   "pipes"       a pass of k >= 2 samples is divided by samples over up to
  *                 min(pipes, k) independent streams whose kernels run
- *                 concurrently, each taking at least queue/2 paths
+ *                 concurrently, each taking at least min(queue/2, 2^22) paths
  *                 (1..4; 1 = one stream; 0 = automatic, the default: 2 for
  *                 scenes traversed from LDS, else 3); read-only
  *                 "pipes_used" reports the last pass's count

@@ -298,9 +298,9 @@ bool Engine::run_tickets(const DevJob *d_jobs, std::vector<DevTicket> tickets,
   if (!traced.empty()) {
     // Pipes: a one-job pass is divided by samples (pipe k traces samples
     // [s_k, s_k+1) into their own radiance slots), anything else runs whole.
-    // Each pipe must still get half a queue of paths: below that the extra
-    // launches cost more than the overlap gains (measured break-even between
-    // 4M and 8M paths per pass at the default 8M-path queue).
+    // Each pipe must still get 4M paths (or half its queue, if smaller):
+    // below that the extra launches cost more than the overlap gains
+    // (measured break-even between 4M and 8M paths per pass).
     std::vector<std::vector<DevTicket>> parts;
     int np = 1;
     if (traced.size() == 1 && host_jobs.size() == 1) {
@@ -311,7 +311,8 @@ bool Engine::run_tickets(const DevJob *d_jobs, std::vector<DevTicket> tickets,
       const uint64_t tree = opt.bvh4 && j.n_nodes4 ? 128ull * j.n_nodes4 : 64ull * j.n_nodes;
       const bool lds = tree + 32ull * j.n_prims <= opt.lds_bvh;
       const int want = opt.pipes > 0 ? opt.pipes : (lds ? 2 : 3);
-      const uint64_t by_size = traced[0].paths / std::max<uint64_t>(1, q_alloc / 2);
+      const uint64_t per_pipe = std::min<uint64_t>(q_alloc / 2, 1ull << 22);
+      const uint64_t by_size = traced[0].paths / std::max<uint64_t>(1, per_pipe);
       np = (int)std::min<uint64_t>({(uint64_t)want, traced[0].spp, by_size});
       np = std::max(np, 1);
     }

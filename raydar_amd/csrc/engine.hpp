@@ -35,7 +35,7 @@ struct DeviceScene {
 
 struct EngineOptions {
   int block = 256;
-  uint32_t queue = 1u << 23;      // paths in flight (two ping-pong queues)
+  uint32_t queue = 1u << 24;      // paths in flight per pipe (two ping-pong queues)
   uint64_t l_capacity = 1ull << 27;  // radiance slots (16 B each)
   int grid = 0;                   // 0: occupancy-derived
   int brute_force = 0;
