@@ -517,9 +517,7 @@ int32_t host_trace(const BuiltScene &b, const float o[3], const float d[3], floa
   const RayPre r = ray_pre(f3(o[0], o[1], o[2]), f3(d[0], d[1], d[2]));
   float bt = 0.f;
   uint32_t bid = kNoHit;
-  const bool finite = std::isfinite(d[0]) && std::isfinite(d[1]) && std::isfinite(d[2]) &&
-                      std::isfinite(o[0]) && std::isfinite(o[1]) && std::isfinite(o[2]);
-  if (!finite || b.nodes.empty()) {
+  if (!walkable(r) || b.nodes.empty()) {
     for (const DevPrim &p : b.prims) test_prim(r, p, &bt, &bid);
   } else {
     const SlabRay s = slab_ray(r, ray_tolerance(r, b.r0));
@@ -602,9 +600,7 @@ int32_t host_trace4(const BuiltScene &b, const float o[3], const float d[3], flo
   const RayPre r = ray_pre(f3(o[0], o[1], o[2]), f3(d[0], d[1], d[2]));
   float bt = 0.f;
   uint32_t bid = kNoHit;
-  const bool finite = std::isfinite(d[0]) && std::isfinite(d[1]) && std::isfinite(d[2]) &&
-                      std::isfinite(o[0]) && std::isfinite(o[1]) && std::isfinite(o[2]);
-  if (!finite || b.nodes4.empty()) {
+  if (!walkable(r) || b.nodes4.empty()) {
     for (const DevPrim &p : b.prims) test_prim(r, p, &bt, &bid);
   } else {
     const SlabRay s = slab_ray(r, ray_tolerance(r, b.r0));

@@ -212,11 +212,9 @@ __device__ __forceinline__ void trace(const DevNode *__restrict__ nodes,
   *bid = kNoHit;
   *bt = 0.0f;
   if (n_prims == 0) return;
-  const bool finite = isfinite(r.ox) && isfinite(r.oy) && isfinite(r.oz) && isfinite(r.dx) &&
-                      isfinite(r.dy) && isfinite(r.dz);
-  if (brute || !finite || n_nodes == 0) {
+  if (brute || !walkable(r) || n_nodes == 0) {
     // the reference's linear scan (cpu.rs:345-349); also the exact answer
-    // for non-finite rays, whose box tests would be meaningless
+    // for rays the box walk cannot represent (walkable())
     for (uint32_t j = 0; j < n_prims; ++j) {
       DevPrim p;
       load_prim(prims, j, &p);
@@ -467,11 +465,9 @@ void k_extend_lanes(IterArgs a, int p) {
         r = ray_pre(o, d);
         bt = 0.0f;
         bid = kNoHit;
-        const bool finite = isfinite(r.ox) && isfinite(r.oy) && isfinite(r.oz) && isfinite(r.dx) &&
-                            isfinite(r.dy) && isfinite(r.dz);
-        if (n_prims == 0 || a.brute_force || !finite || n_nodes == 0) {
+        if (n_prims == 0 || a.brute_force || !walkable(r) || n_nodes == 0) {
           // the reference's linear scan (cpu.rs:345-349), also the exact
-          // answer for non-finite rays; done at once
+          // answer for rays the box walk cannot represent (walkable()); done at once
           for (uint32_t j = 0; j < n_prims; ++j) {
             DevPrim pr;
             load_prim(SINGLE ? prims : J.prims, j, &pr);

@@ -216,6 +216,18 @@ RD_HD float safe_inv(float d) {
   const float m = fabsf(d) < 1e-30f ? copysignf(1e-30f, d) : d;
   return 1.0f / m;
 }
+// Whether the box walk answers exactly as the scan would: finite origin and
+// direction, and no direction component that safe_inv would clamp unless it
+// is an exact zero next to a nonzero component (an axis-parallel ray, whose
+// slabs safe_inv scales correctly).  An all-zero direction or a tiny nonzero
+// component makes the reference's divisions by d (cpu.rs:74-79) overflow
+// differently from the clamped scale; such rays take the linear scan.
+RD_HD bool tiny_nonzero(float d) { return d != 0.0f && fabsf(d) < 1e-30f; }
+RD_HD bool walkable(const RayPre &r) {
+  return isfinite(r.ox) && isfinite(r.oy) && isfinite(r.oz) && isfinite(r.dx) && isfinite(r.dy) &&
+         isfinite(r.dz) && !(r.dx == 0.0f && r.dy == 0.0f && r.dz == 0.0f) &&
+         !tiny_nonzero(r.dx) && !tiny_nonzero(r.dy) && !tiny_nonzero(r.dz);
+}
 RD_HD SlabRay slab_ray(const RayPre &r, float tol) {
   SlabRay s;
   s.ix = safe_inv(r.dx); s.iy = safe_inv(r.dy); s.iz = safe_inv(r.dz);

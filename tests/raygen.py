@@ -63,7 +63,29 @@ def far(scene, n, rng, dist=3000.0):
     return np.concatenate([o, d], 1).astype(np.float32)
 
 
-ALL = (camera, secondary, grazing, axis_parallel, far)
+def degenerate(scene, n, rng):
+    """Directions the box walk cannot represent: all zero (+0 and -0) and
+    tiny nonzero components, which the reference divides by (cpu.rs:74-79),
+    from origins at and around object centres, where cubes report hits at
+    infinity.  The renderer never makes such rays; the closest-hit contract
+    covers any input ray."""
+    objs = scene.objects
+    k = rng.integers(0, len(objs), n)
+    c = objs["center"][k].astype(np.float64)
+    s = objs["size"][k].astype(np.float64)
+    o = c + rng.uniform(-1.0, 1.0, (n, 3)) * s[:, None]
+    d = np.zeros((n, 3))
+    kind = np.arange(n) % 4
+    d[kind == 1] = -0.0
+    sel = kind == 2
+    d[sel] = rng.uniform(-1.0, 1.0, (int(sel.sum()), 3)) * 1e-35
+    sel = kind == 3
+    d[sel] = rng.uniform(-1.0, 1.0, (int(sel.sum()), 3))
+    d[sel, 0] = 1e-33
+    return np.concatenate([o, d], 1).astype(np.float32)
+
+
+ALL = (camera, secondary, grazing, axis_parallel, far, degenerate)
 
 
 def mixed(scene, n_each, seed=0):

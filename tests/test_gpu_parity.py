@@ -26,10 +26,14 @@ REL = 1e-5
 
 
 def _close(a, b, rel=REL):
+    # equal values (hits at infinity included) and NaN on both sides (the
+    # normal at such a hit) match; everything else within rel
     a = np.asarray(a, np.float64)
     b = np.asarray(b, np.float64)
     scale = np.maximum(np.maximum(np.abs(a), np.abs(b)), 1.0)
-    return np.abs(a - b) <= rel * scale
+    with np.errstate(invalid="ignore"):
+        near = np.abs(a - b) <= rel * scale
+    return near | (a == b) | (np.isnan(a) & np.isnan(b))
 
 
 material_zoo = raygen.material_zoo
