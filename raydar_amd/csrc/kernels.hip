@@ -555,22 +555,30 @@ void k_extend_lanes(IterArgs a, int p) {
         }
         // entered inner children in order of entry distance
         const bool i0 = h0 && !l0, i1 = h1 && !l1, i2 = h2 && !l2, i3 = h3 && !l3;
-        float k0 = i0 ? t0 : INFINITY, k1 = i1 ? t1 : INFINITY;
-        float k2 = i2 ? t2 : INFINITY, k3 = i3 ? t3 : INFINITY;
-        int32_t c0 = ch.x, c1 = ch.y, c2 = ch.z, c3 = ch.w;
-        order_pair(k0, c0, k1, c1);
-        order_pair(k2, c2, k3, c3);
-        order_pair(k0, c0, k2, c2);
-        order_pair(k1, c1, k3, c3);
-        order_pair(k1, c1, k2, c2);
         const int cnt = (int)i0 + (int)i1 + (int)i2 + (int)i3;
+        int32_t c0;
+        if (__ballot(cnt > 1) == 0) {
+          // no lane entered two inner children: the entered one (if any) is
+          // the nearest and nothing is pushed
+          c0 = i0 ? ch.x : i1 ? ch.y : i2 ? ch.z : ch.w;
+        } else {
+          float k0 = i0 ? t0 : INFINITY, k1 = i1 ? t1 : INFINITY;
+          float k2 = i2 ? t2 : INFINITY, k3 = i3 ? t3 : INFINITY;
+          int32_t c1 = ch.y, c2 = ch.z, c3 = ch.w;
+          c0 = ch.x;
+          order_pair(k0, c0, k1, c1);
+          order_pair(k2, c2, k3, c3);
+          order_pair(k0, c0, k2, c2);
+          order_pair(k1, c1, k3, c3);
+          order_pair(k1, c1, k2, c2);
 #ifdef RDH_CHECKED
-        sp = (int)chk((uint32_t)sp + 2u, rows, K_STACK, a.ctl) - 2;
-        if (sp < 0) sp = 0;
+          sp = (int)chk((uint32_t)sp + 2u, rows, K_STACK, a.ctl) - 2;
+          if (sp < 0) sp = 0;
 #endif
-        stk[sp * BLOCK] = (StackT)(cnt == 4 ? c3 : cnt == 3 ? c2 : c1);
-        stk[(sp + 1) * BLOCK] = (StackT)(cnt == 4 ? c2 : c1);
-        stk[(sp + 2) * BLOCK] = (StackT)c1;
+          stk[sp * BLOCK] = (StackT)(cnt == 4 ? c3 : cnt == 3 ? c2 : c1);
+          stk[(sp + 1) * BLOCK] = (StackT)(cnt == 4 ? c2 : c1);
+          stk[(sp + 2) * BLOCK] = (StackT)c1;
+        }
         const int top = sp > 0 ? sp - 1 : 0;
         const int32_t popped = (int32_t)stk[top * BLOCK];
         if (cnt > 0) {
