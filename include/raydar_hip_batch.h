@@ -73,8 +73,9 @@ rd_status rd_batch_stats_get(const rd_batch *b, rd_batch_stats *out);
 const char *rd_batch_last_error(const rd_batch *b);
 
 /* Autotuner: times the sample jobs over block {64,128,256} x queue capacity
- * x BVH leaf size {1,2,4,8}, keeps the fastest setting on the batch and
- * returns it. */
+ * {2^20, 2^22, 2^23, 2^24} x BVH leaf size {1,2,4,8}, keeps the fastest
+ * setting on the batch and returns it; default_ms is the time at the
+ * library's defaults (block 256, leaf 1, queue 2^24). */
 typedef struct rd_tune_result {
   int32_t block;
   int32_t leaf;

@@ -990,7 +990,8 @@ rd_status rd_batch_autotune(rd_batch *b, const rd_job_desc *jobs, uint32_t n, rd
         if ((s = run_batch(t, jobs, n)) != RD_OK) return batch_fail(b, s, t->last_error);
         const double ms = t->stats.elapsed_ns / 1e6;
         r.configs_tried++;
-        if (blk == 256 && lf == 4 && q == (1u << 23)) dflt = ms;
+        const rdh::EngineOptions d;  // the library's defaults, for comparison
+        if (blk == d.block && lf == d.leaf && q == d.queue) dflt = ms;
         if (ms < best) {
           best = ms;
           r.block = blk;
