@@ -88,10 +88,11 @@ def cpu_baseline(job, budget_s=10.0):
     }
 
 
-def batched_vs_plain(autotune=True):
+def batched_vs_plain(autotune=True, reps=2):
     """64 clustered scenes (16-2048 primitives, 128^2..1080p, 8/32/128 spp,
     4/8/12 bounces) in one batched wavefront vs 64 sequential plain renders
-    on one reused per-scene renderer (new_frame + render_frame each)."""
+    on one reused per-scene renderer (new_frame + render_frame each); each
+    side warmed by one untimed pass, then timed over `reps` passes."""
     import raydar_amd as ra
     from raydar_amd import scenegen
     from raydar_amd.batch import BatchRenderer, Job
@@ -104,25 +105,36 @@ def batched_vs_plain(autotune=True):
             t0 = time.perf_counter()
             tune = br.autotune(sample)
             out["autotune"] = dict(tune, seconds=round(time.perf_counter() - t0, 2))
-        br.render([Job(specs[0].scene, 1, 4, 0, accumulator=False)])  # warm the launch shapes
-        jobs = [Job(j.scene, j.spp, j.bounces, j.seed, accumulator=False) for j in specs]
-        t0 = time.perf_counter()
-        st = br.render(jobs)
-        dt_b = time.perf_counter() - t0
+        # one untimed pass of the whole batch (launch shapes, graphs, the
+        # kept job buffers), then the mean of `reps` timed passes
+        br.render([Job(j.scene, j.spp, j.bounces, j.seed, accumulator=False) for j in specs])
+        runs = []
+        for _ in range(reps):
+            jobs = [Job(j.scene, j.spp, j.bounces, j.seed, accumulator=False) for j in specs]
+            t0 = time.perf_counter()
+            st = br.render(jobs)
+            runs.append(time.perf_counter() - t0)
+        dt_b = sum(runs) / len(runs)
         out["batched"] = {"seconds": round(dt_b, 4), "mpaths_per_s": round(paths / dt_b / 1e6, 2),
+                          "runs_s": [round(t, 4) for t in runs],
                           "iterations": st["iterations"], "graph_launches": st["graph_launches"],
-                          "options": {k: br.get_option(k) for k in ("block", "queue", "leaf", "pipes")}}
+                          "options": {k: br.get_option(k) for k in ("block", "queue", "leaf", "pipes",
+                                                                    "solo_paths")},
+                          "solo_jobs": br.get_option("solo_jobs")}
     with ra.HipRenderer() as r:
-        r.max_sample_count = 1
-        r.render_frame(specs[0].scene)  # warm-up
-        t0 = time.perf_counter()
-        for j in specs:
-            r.max_sample_count = j.spp
-            r.max_bounces = j.bounces
-            r.set_seed(j.seed)
-            r.render_frame(j.scene)
-        dt_p = time.perf_counter() - t0
-    out["sequential_plain"] = {"seconds": round(dt_p, 4), "mpaths_per_s": round(paths / dt_p / 1e6, 2)}
+        runs = []
+        for k in range(reps + 1):  # the first pass warms up, as for the batch
+            t0 = time.perf_counter()
+            for j in specs:
+                r.max_sample_count = j.spp
+                r.max_bounces = j.bounces
+                r.set_seed(j.seed)
+                r.render_frame(j.scene)
+            if k:
+                runs.append(time.perf_counter() - t0)
+        dt_p = sum(runs) / len(runs)
+    out["sequential_plain"] = {"seconds": round(dt_p, 4), "mpaths_per_s": round(paths / dt_p / 1e6, 2),
+                               "runs_s": [round(t, 4) for t in runs]}
     out["batched_over_plain"] = round(dt_p / dt_b, 3)
     return out
 

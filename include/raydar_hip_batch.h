@@ -64,10 +64,17 @@ void rd_batch_destroy(rd_batch *b);
  * "pipes" (1..4, default 3; settable until the first render): independent
  * engines, each with its own stream, queues and radiance ring; every job is
  * given whole to the one with the fewest pending paths and the host pumps
- * them in turn, so their kernels overlap on the GPU. */
+ * them in turn, so their kernels overlap on the GPU,
+ * "solo_paths" (default 48 Mi = 50331648; 0 = off): a job of at least this
+ * many paths (W * H * samples) without a progress image renders on the solo
+ * lane, a per-scene engine (the rd_render_frame code path: single-job
+ * kernels, small trees in LDS, samples split over pipes) driven by a host
+ * thread of its own, one job after another, while the pipes take the other
+ * jobs; read-only "solo_jobs" (jobs the lane has rendered, cumulative).
+ * Images do not depend on these options. */
 rd_status rd_batch_set_option(rd_batch *b, const char *key, int64_t value);
 rd_status rd_batch_get_option(const rd_batch *b, const char *key, int64_t *value);
-/* Render all jobs to completion in one mixed wavefront. */
+/* Render all jobs to completion in one mixed wavefront (and the solo lane). */
 rd_status rd_batch_render(rd_batch *b, const rd_job_desc *jobs, uint32_t n_jobs);
 rd_status rd_batch_stats_get(const rd_batch *b, rd_batch_stats *out);
 const char *rd_batch_last_error(const rd_batch *b);

@@ -68,6 +68,7 @@ struct rd_renderer {
   uint32_t samples_per_pass = 1;
   uint64_t seed = 0;
   std::string last_error;
+  rdh::DeviceCache cache;  // scene and image buffers reused across frames (outlives eng)
   rdh::Engine eng;
   rdh::DeviceScene scene;
   rdh::DevJob job_h{};
@@ -123,12 +124,12 @@ rd_status snapshot(rd_renderer *r, const rd_scene *s, bool zero_acc) {
   if (!r->eng.upload_scene(built, &r->scene, &err)) return device_fail(r, err);
   const uint32_t W = s->camera.resolution_x, H = s->camera.resolution_y;
   if (W != r->W || H != r->H || !r->acc) {
-    if (r->acc) (void)hipFree(r->acc);
-    if (r->rgba_d) (void)hipFree(r->rgba_d);
+    r->eng.dfree(r->acc);
+    r->eng.dfree(r->rgba_d);
     r->acc = nullptr;
     r->rgba_d = nullptr;
-    if (!r->eng.check(hipMalloc(&r->acc, 16ull * W * H), "hipMalloc(accumulator)", &err) ||
-        !r->eng.check(hipMalloc(&r->rgba_d, 4ull * W * H), "hipMalloc(image)", &err))
+    if (!r->eng.check(r->eng.dmalloc((void **)&r->acc, 16ull * W * H), "hipMalloc(accumulator)", &err) ||
+        !r->eng.check(r->eng.dmalloc((void **)&r->rgba_d, 4ull * W * H), "hipMalloc(image)", &err))
       return device_fail(r, err);
     r->W = W;
     r->H = H;
@@ -257,6 +258,7 @@ rd_status rd_create(const rd_config *config, rd_renderer **out) {
   rd_renderer *r = new (std::nothrow) rd_renderer();
   if (!r) return RD_ERR_OUT_OF_MEMORY;
   if (config) r->cfg = *config;
+  r->eng.cache = &r->cache;
   std::string err;
   if (!r->eng.init(&err)) {
     rdh::set_global_error(err);
@@ -271,8 +273,8 @@ void rd_destroy(rd_renderer *r) {
   if (!r) return;
   (void)hipDeviceSynchronize();
   r->eng.free_scene(&r->scene);
-  if (r->acc) (void)hipFree(r->acc);
-  if (r->rgba_d) (void)hipFree(r->rgba_d);
+  r->eng.dfree(r->acc);
+  r->eng.dfree(r->rgba_d);
   if (r->job_d) (void)hipFree(r->job_d);
   delete r;
 }

@@ -68,13 +68,44 @@ struct SJob {
   Clock::time_point t_submit, t_admit, t_done;
 };
 
-// Device-memory account of one or more streamers: one budget for jobs plus
-// the engines' own buffers.
+// Device-memory account of one or more streamers (and the batch renderer's
+// solo lane, which admits jobs from a thread of its own): one budget for jobs
+// plus the engines' own buffers.
 struct HbmPool {
   uint64_t budget = 200ull << 30;
-  uint64_t fixed = 0;  // engine buffers of every ready streamer
+  uint64_t fixed = 0;  // engine buffers of every ready streamer (and the solo lane)
   uint64_t in_use = 0, peak = 0;
   uint32_t resident = 0, max_resident = 0;
+  mutable std::mutex mu;
+
+  // Takes `bytes` for one job unless that would pass the budget while other
+  // jobs are resident (a job always runs when nothing else is resident).
+  bool reserve(uint64_t bytes) {
+    std::lock_guard<std::mutex> g(mu);
+    if (in_use + bytes + fixed > budget && resident > 0) return false;
+    in_use += bytes;
+    peak = std::max(peak, in_use);
+    resident++;
+    max_resident = std::max(max_resident, resident);
+    return true;
+  }
+  void release(uint64_t bytes) {
+    std::lock_guard<std::mutex> g(mu);
+    in_use -= bytes;
+    resident--;
+  }
+  void add_fixed(int64_t delta) {
+    std::lock_guard<std::mutex> g(mu);
+    fixed += (uint64_t)delta;
+  }
+  bool fits_alone(uint64_t bytes) const {
+    std::lock_guard<std::mutex> g(mu);
+    return bytes + fixed <= budget;
+  }
+  template <class T> T read(T HbmPool::*field) const {
+    std::lock_guard<std::mutex> g(mu);
+    return this->*field;
+  }
 };
 
 }  // namespace
@@ -117,7 +148,7 @@ class Streamer {
     tickets_.reserve(ticket_cap);  // never reallocates: async uploads read from it
     ring_.reset(eng.l_alloc);
     fixed_bytes_ = engine_bytes();
-    pool->fixed += fixed_bytes_;
+    pool->add_fixed((int64_t)fixed_bytes_);
     ready_ = true;
     return eng.check(hipStreamSynchronize(eng.stream), "init", &err);
   }
@@ -168,7 +199,7 @@ class Streamer {
     return j;
   }
 
-  bool fits_alone(const SJob &j) const { return j.bytes + pool->fixed <= pool->budget; }
+  bool fits_alone(const SJob &j) const { return pool->fits_alone(j.bytes); }
 
   void enqueue(std::unique_ptr<SJob> j) {
     queued_.push_back(j.get());
@@ -273,8 +304,8 @@ class Streamer {
   void release_job(SJob *j) {
     if (j->state != 1 && !j->acc && !j->dev.nodes && !j->dev.prims) return;
     eng.free_scene(&j->dev);
-    if (j->acc) (void)hipFree(j->acc);
-    if (j->rgba_d) (void)hipFree(j->rgba_d);
+    eng.dfree(j->acc);
+    eng.dfree(j->rgba_d);
     j->acc = nullptr;
     j->rgba_d = nullptr;
   }
@@ -284,14 +315,23 @@ class Streamer {
       SJob *j = queued_.front();
       if (free_slots_.empty()) break;
       // wait for room unless nothing is resident (a job that fits alone always runs)
-      if (pool->in_use + j->bytes + pool->fixed > pool->budget && pool->resident > 0) break;
+      if (!pool->reserve(j->bytes)) break;
       queued_.pop_front();
+      // until the job is resident a failure gives its reservation back
+      struct Unreserve {
+        HbmPool *p;
+        uint64_t b;
+        bool keep = false;
+        ~Unreserve() {
+          if (!keep) p->release(b);
+        }
+      } unreserve{pool, j->bytes};
       j->slot = free_slots_.back();
       free_slots_.pop_back();
       const uint64_t npix = (uint64_t)j->W * j->H;
       if (!eng.upload_scene(j->built, &j->dev, &err)) return false;
-      if (!eng.check(hipMalloc(&j->acc, 16 * npix), "hipMalloc(accumulator)", &err)) return false;
-      if (!eng.check(hipMalloc(&j->rgba_d, 4 * npix), "hipMalloc(image)", &err)) return false;
+      if (!eng.check(eng.dmalloc((void **)&j->acc, 16 * npix), "hipMalloc(accumulator)", &err)) return false;
+      if (!eng.check(eng.dmalloc((void **)&j->rgba_d, 4 * npix), "hipMalloc(image)", &err)) return false;
       if (!eng.check(launch_zero_acc(j->acc, 4 * npix, eng.stream), "zero accumulator", &err)) return false;
       // the host mirror entry outlives the asynchronous copy
       DevJob &dj = h_jobs_[j->slot];
@@ -305,10 +345,7 @@ class Streamer {
       j->state = 1;
       j->t_admit = Clock::now();
       resident_.push_back(j);
-      pool->in_use += j->bytes;
-      pool->peak = std::max(pool->peak, pool->in_use);
-      pool->resident++;
-      pool->max_resident = std::max(pool->max_resident, pool->resident);
+      unreserve.keep = true;
     }
     return true;
   }
@@ -529,8 +566,7 @@ class Streamer {
         j->state = 2;
         release_job(j);
         free_slots_.push_back(j->slot);
-        pool->in_use -= j->bytes;
-        pool->resident--;
+        pool->release(j->bytes);
         resident_.erase(std::find(resident_.begin(), resident_.end(), j));
       }
     }
@@ -547,6 +583,7 @@ class Streamer {
 // (memory-bound) kernels share the CUs.
 class StreamerGroup {
  public:
+  DeviceCache cache;  // job buffers of every member (declared first: outlives them)
   std::vector<std::unique_ptr<Streamer>> m;
   std::string err;
   uint64_t pumps = 0;
@@ -565,10 +602,9 @@ class StreamerGroup {
   }
   bool resize(int n) {
     if ((int)m.size() > n) {
+      for (size_t i = n; i < m.size(); ++i) pool_.add_fixed(-(int64_t)m[i]->engine_bytes_if_ready());
       m.resize(n);
       launched_.resize(n);
-      pool_.fixed = 0;
-      for (const auto &s : m) pool_.fixed += s->engine_bytes_if_ready();
       return true;
     }
     return grow(n);
@@ -581,8 +617,12 @@ class StreamerGroup {
   }
 
   // One HBM budget for all members (jobs plus every member's own buffers).
-  void set_budget(uint64_t total) { pool_.budget = total; }
-  uint64_t budget() const { return pool_.budget; }
+  void set_budget(uint64_t total) {
+    std::lock_guard<std::mutex> g(pool_.mu);
+    pool_.budget = total;
+  }
+  uint64_t budget() const { return pool_.read(&HbmPool::budget); }
+  HbmPool &pool() { return pool_; }
 
   std::unique_ptr<SJob> make_job(const rd_job_desc &d, int leaf, std::string *e) {
     return m[0]->make_job(d, leaf, e);
@@ -651,10 +691,10 @@ class StreamerGroup {
   uint64_t iterations() const { return sum([](const Streamer &s) { return s.iterations(); }); }
   uint64_t graph_launches() const { return sum([](const Streamer &s) { return s.graph_launches; }); }
   uint64_t paths_done() const { return sum([](const Streamer &s) { return s.paths_done; }); }
-  uint64_t hbm_in_use() const { return pool_.in_use; }
-  uint64_t hbm_peak() const { return pool_.peak; }
+  uint64_t hbm_in_use() const { return pool_.read(&HbmPool::in_use); }
+  uint64_t hbm_peak() const { return pool_.read(&HbmPool::peak); }
   uint64_t engine_bytes() const { return sum([](const Streamer &s) { return s.engine_bytes(); }); }
-  uint32_t max_resident() const { return pool_.max_resident; }
+  uint32_t max_resident() const { return pool_.read(&HbmPool::max_resident); }
   uint32_t n_resident() const { return (uint32_t)sum([](const Streamer &s) { return s.n_resident(); }); }
   uint32_t n_queued() const { return (uint32_t)sum([](const Streamer &s) { return s.n_queued(); }); }
 
@@ -682,9 +722,206 @@ class StreamerGroup {
         return false;
       }
       s->pool = &pool_;
+      s->eng.cache = &cache;
       m.push_back(std::move(s));
       launched_.push_back(0);
     }
+    return true;
+  }
+};
+
+// The batch renderer's solo lane.  A job of at least `min_paths` paths fills
+// the GPU by itself, and the single-scene kernels render it faster than the
+// shared queues do: the job's fields are launch constants, a tree of up to
+// 14 KB is traversed from LDS, and the samples are split over pipes.  Such
+// jobs are handed to this lane, which renders them one after another on a
+// host thread of its own with a per-scene engine (the plain renderer's code
+// path), while the streamers' shared queues take the smaller jobs; the two
+// sets of kernels share the CUs.  A job's image does not depend on which
+// side renders it: each pixel's samples are drawn from the same counters and
+// accumulated in sample order either way.
+constexpr uint64_t kSoloPaths = 48ull << 20;
+
+class SoloLane {
+ public:
+  Engine eng;
+  uint64_t min_paths = kSoloPaths;  // 0: every job goes to the streamers
+  uint64_t jobs_done = 0, paths_done = 0;
+
+  ~SoloLane() {
+    stop();
+    if (d_job_) (void)hipFree(d_job_);
+  }
+
+  // Whether a job goes to this lane: large enough, no progress image (the
+  // lane renders a job in one go) and samples that fit the radiance arena.
+  bool wants(const rd_job_desc &d) const {
+    if (min_paths == 0 || d.progress_rgba || !d.scene) return false;
+    const uint64_t npix = (uint64_t)d.scene->camera.resolution_x * d.scene->camera.resolution_y;
+    return npix * d.samples >= min_paths && npix <= EngineOptions().l_capacity;
+  }
+
+  // Queues a job, starting the lane's thread for this batch call if needed.
+  // The job stays owned by the caller until finish() or stop() returns.
+  bool push(SJob *j, HbmPool *pool, DeviceCache *cache, const EngineOptions &group_opt,
+            std::string *err) {
+    if (!th_.joinable()) {
+      if (!inited_) {
+        if (!eng.init(err)) return false;
+        inited_ = true;
+      }
+      // launch-shape and traversal options follow the batch; the queue and
+      // the radiance arena keep the per-scene renderer's sizes
+      const EngineOptions d;
+      eng.opt = group_opt;
+      eng.opt.queue = d.queue;
+      eng.opt.l_capacity = d.l_capacity;
+      eng.opt.lds_bvh = d.lds_bvh;
+      eng.opt.pipes = 0;
+      eng.cache = cache;
+      pool_ = pool;
+      closed_ = false;
+      stop_ = false;
+      failed_ = false;
+      err_.clear();
+      th_ = std::thread(&SoloLane::run, this);
+    }
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      q_.push_back(j);
+    }
+    cv_.notify_one();
+    return true;
+  }
+
+  bool failed() const { return failed_.load(); }
+
+  // Waits until every queued job is rendered.  False (with the message) if a
+  // job failed; jobs queued behind a failed one are not rendered.
+  bool finish(std::string *err) {
+    join();
+    if (failed_) *err = err_;
+    return !failed_;
+  }
+
+  // Drops the jobs not yet started and waits for the current one.
+  void stop() {
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      q_.clear();
+      stop_ = true;
+    }
+    join();
+  }
+
+ private:
+  std::thread th_;
+  std::mutex mu_;
+  std::condition_variable cv_;
+  std::deque<SJob *> q_;
+  bool closed_ = false;
+  std::atomic<bool> stop_{false}, failed_{false};
+  std::string err_;
+  HbmPool *pool_ = nullptr;
+  DevJob *d_job_ = nullptr;
+  bool inited_ = false;
+  uint64_t accounted_ = 0;  // engine bytes already counted in the pool
+
+  void join() {
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      closed_ = true;
+    }
+    cv_.notify_all();
+    if (th_.joinable()) th_.join();
+  }
+
+  void run() {
+    if (!eng.check(hipSetDevice(eng.device), "hipSetDevice", &err_)) failed_ = true;
+    for (;;) {
+      SJob *j = nullptr;
+      {
+        std::unique_lock<std::mutex> lk(mu_);
+        cv_.wait(lk, [&] { return !q_.empty() || closed_; });
+        if (q_.empty()) return;
+        j = q_.front();
+        q_.pop_front();
+      }
+      if (failed_ || stop_) continue;
+      if (!render(*j)) failed_ = true;
+    }
+  }
+
+  bool render(SJob &j) {
+    // admission under the group's budget, as for the streamers' jobs
+    while (!pool_->reserve(j.bytes)) {
+      if (stop_) return true;
+      std::this_thread::sleep_for(std::chrono::microseconds(200));
+    }
+    const uint64_t npix = (uint64_t)j.W * j.H;
+    const uint64_t fit = std::min<uint64_t>(eng.opt.l_capacity / npix, 0xffffffffull / npix);
+    std::string ignored;
+    bool ok = fit > 0;
+    if (!ok) err_ = "image larger than the radiance arena";
+    j.t_admit = Clock::now();
+    ok = ok && eng.upload_scene(j.built, &j.dev, &err_) &&
+         eng.check(eng.dmalloc((void **)&j.acc, 16 * npix), "hipMalloc(accumulator)", &err_) &&
+         eng.check(eng.dmalloc((void **)&j.rgba_d, 4 * npix), "hipMalloc(image)", &err_) &&
+         eng.check(launch_zero_acc(j.acc, 4 * npix, eng.stream), "zero accumulator", &err_);
+    if (ok && !d_job_) ok = eng.check(hipMalloc(&d_job_, sizeof(DevJob)), "hipMalloc(job)", &err_);
+    DevJob h{};
+    if (ok) {
+      fill_job(j.scene, j.dev, j.acc, j.bounces, j.seed, &h);
+      ok = eng.check(hipMemcpyAsync(d_job_, &h, sizeof(DevJob), hipMemcpyHostToDevice, eng.stream),
+                     "upload job", &err_);
+    }
+    const std::vector<DevJob> host{h};
+    for (uint32_t done = 0; ok && done < j.spp;) {
+      // passes as large as the radiance arena allows, in sample order
+      DevTicket t{};
+      t.job = 0;
+      t.sample_base = done;
+      t.spp = (uint32_t)std::min<uint64_t>(j.spp - done, fit);
+      t.paths = (uint32_t)(t.spp * npix);
+      ok = eng.run_tickets(d_job_, {t}, host, &err_);
+      done += t.spp;
+    }
+    if (ok) {
+      const int grid = (int)std::max<uint64_t>(1, std::min<uint64_t>((npix + 255) / 256, 8192));
+      ok = eng.check(launch_tonemap(d_job_, (uint32_t)npix, (float)j.spp, j.rgba_d, eng.counters,
+                                    false, grid, eng.stream),
+                     "tonemap", &err_) &&
+           (!j.out_rgba || eng.check(hipMemcpyAsync(j.out_rgba, j.rgba_d, 4 * npix,
+                                                    hipMemcpyDeviceToHost, eng.stream),
+                                     "read image", &err_)) &&
+           (!j.out_acc || eng.check(hipMemcpyAsync(j.out_acc, j.acc, 16 * npix,
+                                                   hipMemcpyDeviceToHost, eng.stream),
+                                    "read accumulator", &err_));
+    }
+    // nothing may be in flight when the job's buffers are freed
+    const bool synced = eng.check(hipStreamSynchronize(eng.stream), "render", ok ? &err_ : &ignored);
+    ok = ok && synced;
+    eng.free_scene(&j.dev);
+    eng.dfree(j.acc);
+    eng.dfree(j.rgba_d);
+    j.acc = nullptr;
+    j.rgba_d = nullptr;
+    j.built = BuiltScene();
+    pool_->release(j.bytes);
+    // the engine keeps its buffers between jobs: count them once
+    const uint64_t held = eng.device_bytes();
+    if (held > accounted_) {
+      pool_->add_fixed((int64_t)(held - accounted_));
+      accounted_ = held;
+    }
+    if (!ok) {
+      j.state = 3;
+      return false;
+    }
+    j.state = 2;
+    j.t_done = Clock::now();
+    ++jobs_done;
+    paths_done += npix * j.spp;
     return true;
   }
 };
@@ -694,6 +931,7 @@ class StreamerGroup {
 // ------------------------------------------------------------- batch ABI ---
 struct rd_batch {
   rdh::StreamerGroup st;
+  rdh::SoloLane solo;  // declared after st: its thread stops before the group goes
   std::string last_error;
   bool failed = false;  // a device error left work behind: the engines are unusable
   rd_batch_stats stats{};
@@ -829,6 +1067,7 @@ rd_status run_batch(rd_batch *b, const rd_job_desc *jobs, uint32_t n) {
   }
   const auto t0 = std::chrono::steady_clock::now();
   const uint64_t it0 = st.iterations(), g0 = st.graph_launches(), p0 = st.pumps, paths0 = st.paths_done();
+  const uint64_t solo_it0 = b->solo.eng.iterations, solo_paths0 = b->solo.paths_done;
   // Host-side scene digestion (BVH builds) runs on a thread pool and
   // overlaps the GPU: jobs join the work list in submission order as soon as
   // they are built.
@@ -851,6 +1090,7 @@ rd_status run_batch(rd_batch *b, const rd_job_desc *jobs, uint32_t n) {
   std::vector<std::thread> pool;
   for (uint32_t t = 0; t < std::min(hw, n); ++t) pool.emplace_back(worker);
   std::vector<uint64_t> ids;
+  std::vector<std::unique_ptr<rdh::SJob>> solo_jobs;  // owned here until the lane has finished
   uint32_t enq = 0;
   rd_status status = RD_OK;
   while (status == RD_OK) {
@@ -860,11 +1100,24 @@ rd_status run_batch(rd_batch *b, const rd_job_desc *jobs, uint32_t n) {
         break;
       }
       built[enq]->id = b->next_id++;
-      ids.push_back(built[enq]->id);
-      st.enqueue(std::move(built[enq]));
+      if (b->solo.wants(jobs[enq])) {
+        solo_jobs.push_back(std::move(built[enq]));
+        std::string e;
+        if (!b->solo.push(solo_jobs.back().get(), &st.pool(), &st.cache, st.first().eng.opt, &e)) {
+          status = batch_fail(b, RD_ERR_DEVICE, e);
+          break;
+        }
+      } else {
+        ids.push_back(built[enq]->id);
+        st.enqueue(std::move(built[enq]));
+      }
       ++enq;
     }
     if (status != RD_OK) break;
+    if (b->solo.failed()) {
+      status = RD_ERR_DEVICE;  // the lane's message is taken when it has stopped
+      break;
+    }
     if (st.idle()) {
       if (enq == n) break;
       std::this_thread::yield();  // waiting for the next scene to be built
@@ -875,6 +1128,24 @@ rd_status run_batch(rd_batch *b, const rd_job_desc *jobs, uint32_t n) {
   }
   next = n;  // stop workers that have not started a job
   for (auto &t : pool) t.join();
+  // the lane's remaining jobs: all of them on success, none after a failure
+  bool lane_failed = false;
+  if (status == RD_OK) {
+    std::string e;
+    if (!b->solo.finish(&e)) {
+      status = batch_fail(b, RD_ERR_DEVICE, "solo lane: " + e);
+      lane_failed = true;
+    }
+  } else {
+    lane_failed = b->solo.failed();
+    b->solo.stop();
+    if (lane_failed) {
+      std::string e;
+      (void)b->solo.finish(&e);
+      b->last_error = "solo lane: " + e;
+    }
+  }
+  if (lane_failed) b->failed = true;  // the lane's engine may hold unfinished work
   // after a failure, drain what was already submitted so no job is left half-rendered
   while (status != RD_OK && !st.idle() && st.pump()) {
   }
@@ -892,10 +1163,10 @@ rd_status run_batch(rd_batch *b, const rd_job_desc *jobs, uint32_t n) {
   b->stats.build_ns = build_ns.load();
   b->stats.host_threads = std::min(hw, n);
   b->stats.elapsed_ns = rdh::ns_between(t0, std::chrono::steady_clock::now());
-  b->stats.iterations = st.iterations() - it0;
+  b->stats.iterations = st.iterations() - it0 + (b->solo.eng.iterations - solo_it0);
   b->stats.graph_launches = st.graph_launches() - g0;
   b->stats.pumps = st.pumps - p0;
-  b->stats.paths = st.paths_done() - paths0;
+  b->stats.paths = st.paths_done() - paths0 + (b->solo.paths_done - solo_paths0);
   return RD_OK;
 }
 
@@ -928,17 +1199,23 @@ void rd_batch_destroy(rd_batch *b) {
 rd_status rd_batch_set_option(rd_batch *b, const char *key, int64_t value) {
   if (!b || !key) return RD_ERR_INVALID_ARGUMENT;
   std::string e;
+  if (std::string(key) == "solo_paths") {
+    if (value < 0) return batch_fail(b, RD_ERR_INVALID_ARGUMENT, "solo_paths must be >= 0");
+    b->solo.min_paths = (uint64_t)value;
+    return RD_OK;
+  }
   if (!group_option(b->st, key, value, &e)) return batch_fail(b, RD_ERR_INVALID_ARGUMENT, e);
   return RD_OK;
 }
 
 rd_status rd_batch_get_option(const rd_batch *b, const char *key, int64_t *value) {
   if (!b || !key || !value) return RD_ERR_INVALID_ARGUMENT;
-  if (std::string(key) == "pipes") {
-    *value = (int64_t)b->st.m.size();
-    return RD_OK;
-  }
-  return engine_option_get(b->st.first(), key, value) ? RD_OK : RD_ERR_INVALID_ARGUMENT;
+  const std::string k = key;
+  if (k == "pipes") *value = (int64_t)b->st.m.size();
+  else if (k == "solo_paths") *value = (int64_t)b->solo.min_paths;
+  else if (k == "solo_jobs") *value = (int64_t)b->solo.jobs_done;
+  else return engine_option_get(b->st.first(), key, value) ? RD_OK : RD_ERR_INVALID_ARGUMENT;
+  return RD_OK;
 }
 
 rd_status rd_batch_render(rd_batch *b, const rd_job_desc *jobs, uint32_t n) {
@@ -973,6 +1250,7 @@ rd_status rd_batch_autotune(rd_batch *b, const rd_job_desc *jobs, uint32_t n, rd
     if (s != RD_OK) return batch_fail(b, s, rd_last_error(nullptr));
     std::unique_ptr<rd_batch, void (*)(rd_batch *)> guard(t, rd_batch_destroy);
     if (!t->st.resize((int)b->st.m.size())) return batch_fail(b, RD_ERR_DEVICE, t->st.err);
+    t->solo.min_paths = b->solo.min_paths;
     const int use_graph = b->st.first().use_graph;
     t->st.each([&](rdh::Streamer &s) {
       s.eng.opt = base;

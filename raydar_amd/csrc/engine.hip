@@ -127,6 +127,14 @@ bool Engine::ensure_buffers(std::string *err) {
   return true;
 }
 
+uint64_t Engine::device_bytes() const {
+  // two queues of 14 words per slot plus the 8-byte hit record
+  uint64_t b = (uint64_t)kSegments * seg_cap * (2 * 56 + 8) + l_alloc * sizeof(float4) +
+               (uint64_t)ticket_alloc * (sizeof(DevTicket) + 8);
+  for (const auto &e : aux_) b += e->device_bytes();
+  return b;
+}
+
 bool Engine::ensure_tickets(uint32_t n, std::string *err) {
   if (n <= ticket_alloc) return true;
   uint32_t cap = std::max<uint32_t>(n, 2 * ticket_alloc);
@@ -140,6 +148,85 @@ bool Engine::ensure_tickets(uint32_t n, std::string *err) {
   return true;
 }
 
+size_t DeviceCache::size_class(size_t bytes) {
+  if (bytes <= 4096) return 4096;
+  int e = 63 - __builtin_clzll((unsigned long long)bytes);
+  const size_t step = (size_t)1 << (e - 3);
+  return (bytes + step - 1) / step * step;
+}
+
+hipError_t DeviceCache::alloc(void **p, size_t bytes) {
+  const size_t c = size_class(bytes);
+  {
+    std::lock_guard<std::mutex> g(mu_);
+    auto it = kept_.find(c);
+    if (it != kept_.end()) {
+      *p = it->second;
+      kept_.erase(it);
+      kept_bytes_ -= c;
+      ++hits;
+      return hipSuccess;
+    }
+    ++misses;
+  }
+  const hipError_t e = hipMalloc(p, c);
+  if (e != hipSuccess) return e;
+  std::lock_guard<std::mutex> g(mu_);
+  class_of_[*p] = c;
+  return hipSuccess;
+}
+
+void DeviceCache::free(void *p) {
+  if (!p) return;
+  std::vector<void *> drop;
+  {
+    std::lock_guard<std::mutex> g(mu_);
+    auto it = class_of_.find(p);
+    if (it == class_of_.end()) return;
+    kept_.emplace(it->second, p);
+    kept_bytes_ += it->second;
+    // over the limit: give back the largest kept buffers down to half of it
+    if (kept_bytes_ > keep_max)
+      while (kept_bytes_ > keep_max / 2 && !kept_.empty()) {
+        auto last = std::prev(kept_.end());
+        kept_bytes_ -= last->first;
+        drop.push_back(last->second);
+        class_of_.erase(last->second);
+        kept_.erase(last);
+      }
+  }
+  for (void *q : drop) (void)hipFree(q);
+}
+
+void DeviceCache::trim() {
+  std::vector<void *> drop;
+  {
+    std::lock_guard<std::mutex> g(mu_);
+    for (auto &kv : kept_) {
+      drop.push_back(kv.second);
+      class_of_.erase(kv.second);
+    }
+    kept_.clear();
+    kept_bytes_ = 0;
+  }
+  for (void *q : drop) (void)hipFree(q);
+}
+
+uint64_t DeviceCache::kept_bytes() const {
+  std::lock_guard<std::mutex> g(mu_);
+  return kept_bytes_;
+}
+
+hipError_t Engine::dmalloc(void **p, size_t bytes) {
+  return cache ? cache->alloc(p, bytes) : hipMalloc(p, bytes);
+}
+
+void Engine::dfree(void *p) {
+  if (!p) return;
+  if (cache) cache->free(p);
+  else (void)hipFree(p);
+}
+
 bool Engine::upload_scene(const BuiltScene &b, DeviceScene *out, std::string *err) {
   free_scene(out);
   out->n_nodes = (uint32_t)b.nodes.size();
@@ -151,10 +238,10 @@ bool Engine::upload_scene(const BuiltScene &b, DeviceScene *out, std::string *er
   const size_t nb = sizeof(DevNode) * b.nodes.size(), pb = sizeof(DevPrim) * b.prims.size(),
                ob = sizeof(DevObject) * b.objects.size(), n4 = sizeof(DevNode4) * b.nodes4.size();
   out->bytes = nb + pb + ob + n4;
-  if (nb && !check(hipMalloc(&out->nodes, nb), "hipMalloc(bvh)", err)) return false;
-  if (n4 && !check(hipMalloc(&out->nodes4, n4), "hipMalloc(bvh4)", err)) return false;
-  if (pb && !check(hipMalloc(&out->prims, pb), "hipMalloc(prims)", err)) return false;
-  if (ob && !check(hipMalloc(&out->objects, ob), "hipMalloc(objects)", err)) return false;
+  if (nb && !check(dmalloc((void **)&out->nodes, nb), "hipMalloc(bvh)", err)) return false;
+  if (n4 && !check(dmalloc((void **)&out->nodes4, n4), "hipMalloc(bvh4)", err)) return false;
+  if (pb && !check(dmalloc((void **)&out->prims, pb), "hipMalloc(prims)", err)) return false;
+  if (ob && !check(dmalloc((void **)&out->objects, ob), "hipMalloc(objects)", err)) return false;
   if (nb && !check(hipMemcpyAsync(out->nodes, b.nodes.data(), nb, hipMemcpyHostToDevice, stream),
                    "upload bvh", err))
     return false;
@@ -172,10 +259,10 @@ bool Engine::upload_scene(const BuiltScene &b, DeviceScene *out, std::string *er
 }
 
 void Engine::free_scene(DeviceScene *s) {
-  if (s->nodes) (void)hipFree(s->nodes);
-  if (s->nodes4) (void)hipFree(s->nodes4);
-  if (s->prims) (void)hipFree(s->prims);
-  if (s->objects) (void)hipFree(s->objects);
+  dfree(s->nodes);
+  dfree(s->nodes4);
+  dfree(s->prims);
+  dfree(s->objects);
   *s = DeviceScene();
 }
 

@@ -10,8 +10,11 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "bvh.hpp"
@@ -31,6 +34,31 @@ struct DeviceScene {
   float r0 = 0.f;
   uint32_t depth = 0;
   size_t bytes = 0;
+};
+
+// Device buffers given back by finished jobs, kept for later jobs.  hipFree
+// waits for the work queued on every stream of the device, so freeing a
+// job's buffers while other pipes have kernels queued would stall the host
+// thread and let those queues run dry.  Buffers are rounded up to size
+// classes (each power of two split in eight steps) and reused by class; they
+// are freed for good by trim(), when more than `keep_max` bytes are kept, and
+// on destruction.  Thread-safe.
+class DeviceCache {
+ public:
+  ~DeviceCache() { trim(); }
+  hipError_t alloc(void **p, size_t bytes);
+  void free(void *p);  // p from alloc(), or nullptr
+  void trim();
+  uint64_t kept_bytes() const;
+  uint64_t hits = 0, misses = 0;
+  uint64_t keep_max = 4ull << 30;
+  static size_t size_class(size_t bytes);
+
+ private:
+  mutable std::mutex mu_;
+  std::unordered_map<void *, size_t> class_of_;  // every buffer handed out
+  std::multimap<size_t, void *> kept_;
+  uint64_t kept_bytes_ = 0;
 };
 
 struct EngineOptions {
@@ -63,6 +91,10 @@ class Engine {
 
   bool upload_scene(const BuiltScene &b, DeviceScene *out, std::string *err);
   void free_scene(DeviceScene *s);
+  // Job buffers: from `cache` when one is set, else hipMalloc / hipFree.
+  DeviceCache *cache = nullptr;
+  hipError_t dmalloc(void **p, size_t bytes);
+  void dfree(void *p);
 
   // Runs one batch of tickets to completion (all paths finished), then
   // accumulates every ticket into its job's accumulator in ticket order.
@@ -112,6 +144,9 @@ class Engine {
   // been generated and at most `bound` are in flight (kNoBound otherwise).
   bool launch_iteration(const IterArgs &a, int parity, std::string *err, uint32_t bound = kNoBound);
   bool ensure_tickets(uint32_t n, std::string *err);
+  // Device bytes held by the queues, hit buffers, radiance arena and ticket
+  // tables of this engine and its auxiliary pipes.
+  uint64_t device_bytes() const;
 
  private:
   bool timed(int which, std::string *err, hipError_t (*fn)(const IterArgs &, int,

@@ -188,15 +188,59 @@ def test_autotune_returns_a_valid_setting():
     np.testing.assert_array_equal(out[0].acc, _oracle_acc(scenegen.plain_small(0), 2, 4, 0))
 
 
-def test_batch_full_size_job_among_small_ones():
+_BIG_REF = {}
+
+
+@pytest.mark.parametrize("solo_paths", [0, 1 << 21])
+def test_batch_full_size_job_among_small_ones(solo_paths):
     # a full 1080p job (2 spp, 4.1 M paths) sharing three pipes with small
-    # jobs; it must come out as rendered alone, equal to the oracle
+    # jobs, or (solo_paths 2^21) rendered by the solo lane beside them; it
+    # must come out as rendered alone, equal to the oracle
     big = (scenegen.plain_1080p(1), 2, 12, 21)
     specs = [big] + _jobs()[:3]
     jobs = [Job(s, spp, b, seed) for s, spp, b, seed in specs]
-    with BatchRenderer() as br:
+    with BatchRenderer(solo_paths=solo_paths) as br:
         br.render(jobs)
-    ref = oracle.render(*big, threads=max(1, min(16, os.cpu_count() or 1)))
+        assert br.get_option("solo_jobs") == (1 if solo_paths else 0)
+    if "big" not in _BIG_REF:
+        _BIG_REF["big"] = oracle.render(*big, threads=max(1, min(16, os.cpu_count() or 1)))
+    ref = _BIG_REF["big"]
     np.testing.assert_array_equal(jobs[0].acc.view(np.uint32), ref.view(np.uint32))
+    np.testing.assert_array_equal(jobs[0].rgba.reshape(-1), oracle.tonemap(ref, 2))
     for job, (s, spp, b, seed) in zip(jobs[1:], specs[1:]):
+        np.testing.assert_array_equal(job.acc.view(np.uint32), _oracle_acc(s, spp, b, seed).view(np.uint32))
+
+
+def test_batch_solo_lane_routing_and_results():
+    # jobs of at least solo_paths paths (and no progress image) render on the
+    # solo lane's per-scene engine, the rest on the shared queues; every
+    # image equals the oracle's either way
+    specs = _jobs()
+    with BatchRenderer() as br:
+        assert br.get_option("solo_paths") == 48 << 20
+        with pytest.raises(ra.RaydarError) as e:
+            br.set_option("solo_paths", -1)
+        assert e.value.status == abi.RD_ERR_INVALID_ARGUMENT
+        for threshold in (1, 20000):
+            br.set_option("solo_paths", threshold)
+            jobs = [Job(s, spp, b, seed) for s, spp, b, seed in specs]
+            # above the threshold but with a progress image: stays on the queues
+            jobs.append(Job(specs[0][0], 2, 6, 31, progress=True))
+            before = br.get_option("solo_jobs")
+            st = br.render(jobs)
+            assert st["paths"] == sum(j.paths for j in jobs)
+            lane = sum(1 for j in jobs[:-1] if j.paths >= threshold)
+            assert 0 < lane < len(jobs)
+            assert br.get_option("solo_jobs") - before == lane
+            for job, (s, spp, b, seed) in zip(jobs, specs):
+                ref = _oracle_acc(s, spp, b, seed)
+                np.testing.assert_array_equal(job.acc.view(np.uint32), ref.view(np.uint32))
+                np.testing.assert_array_equal(job.rgba.reshape(-1), oracle.tonemap(ref, spp))
+            ref = _oracle_acc(specs[0][0], 2, 6, 31)
+            np.testing.assert_array_equal(jobs[-1].acc.view(np.uint32), ref.view(np.uint32))
+        # a second call reuses the lane's engine
+        br.set_option("solo_paths", 1)
+        again = [Job(s, spp, b, seed) for s, spp, b, seed in specs[:2]]
+        br.render(again)
+    for job, (s, spp, b, seed) in zip(again, specs[:2]):
         np.testing.assert_array_equal(job.acc.view(np.uint32), _oracle_acc(s, spp, b, seed).view(np.uint32))
