@@ -205,8 +205,9 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--autotune", action="store_true",
                     help="autotune the batched renderer on a sample of the jobs first")
-    ap.add_argument("--serve-seconds", type=float, default=0.0,
-                    help="length of a Poisson serving run (default 0: skipped)")
+    ap.add_argument("--serve-seconds", type=float, default=1.5,
+                    help="length of the Poisson serving run (default 1.5 s, the length the "
+                         "GPU tests run; 0: skipped)")
     ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--option", action="append", default=[], help="key=value renderer option")
     ap.add_argument("--timed-only", action="store_true",
