@@ -88,11 +88,13 @@ def cpu_baseline(job, budget_s=10.0):
     }
 
 
-def batched_vs_plain(autotune=True, reps=2):
+def batched_vs_plain(autotune=True, reps=2, gap=0.0):
     """64 clustered scenes (16-2048 primitives, 128^2..1080p, 8/32/128 spp,
     4/8/12 bounces) in one batched wavefront vs 64 sequential plain renders
     on one reused per-scene renderer (new_frame + render_frame each); each
-    side warmed by one untimed pass, then timed over `reps` passes."""
+    side warmed by one untimed pass, then timed over `reps` passes.  `gap`:
+    seconds of idle GPU before every pass, so that a kernel trace of the run
+    splits into its passes."""
     import raydar_amd as ra
     from raydar_amd import scenegen
     from raydar_amd.batch import BatchRenderer, Job
@@ -107,10 +109,12 @@ def batched_vs_plain(autotune=True, reps=2):
             out["autotune"] = dict(tune, seconds=round(time.perf_counter() - t0, 2))
         # one untimed pass of the whole batch (launch shapes, graphs, the
         # kept job buffers), then the mean of `reps` timed passes
+        time.sleep(gap)
         br.render([Job(j.scene, j.spp, j.bounces, j.seed, accumulator=False) for j in specs])
         runs = []
         for _ in range(reps):
             jobs = [Job(j.scene, j.spp, j.bounces, j.seed, accumulator=False) for j in specs]
+            time.sleep(gap)
             t0 = time.perf_counter()
             st = br.render(jobs)
             runs.append(time.perf_counter() - t0)
@@ -124,6 +128,7 @@ def batched_vs_plain(autotune=True, reps=2):
     with ra.HipRenderer() as r:
         runs = []
         for k in range(reps + 1):  # the first pass warms up, as for the batch
+            time.sleep(gap)
             t0 = time.perf_counter()
             for j in specs:
                 r.max_sample_count = j.spp
@@ -203,6 +208,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=10.0)
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--batched-only", action="store_true",
+                    help="only the batched-vs-plain comparison (for runs under a profiler)")
+    ap.add_argument("--phase-gap", type=float, default=0.0,
+                    help="seconds of idle GPU before each pass of the batched comparison")
     ap.add_argument("--autotune", action="store_true",
                     help="autotune the batched renderer on a sample of the jobs first")
     ap.add_argument("--serve-seconds", type=float, default=1.5,
@@ -222,6 +231,11 @@ def main():
         sys.exit(2)
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy)
+        return
+
+    if args.batched_only:
+        print(json.dumps({"batched": batched_vs_plain(autotune=args.autotune, gap=args.phase_gap)}),
+              flush=True)
         return
 
     import raydar_amd as ra
