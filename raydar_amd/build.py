@@ -33,7 +33,9 @@ HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wall",
 ]
-CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wextra",
+# -O3: the BVH builder runs 2.4x faster than at -O2 (same trees: no fast-math,
+# so no reassociation)
+CXX_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wextra",
              "-Wno-unknown-pragmas"]
 
 HIP_SOURCES = ["kernels.hip", "engine.hip", "api.hip", "batch.hip"]

@@ -151,10 +151,34 @@ struct Builder {
       for (int a = 0; a < 3; ++a) {
         const float ext = cb.hi[a] - cb.lo[a];
         bn.lo[a] = cb.lo[a];
-        bn.scale[a] = ext > 0.f ? (float)kBins * (1.0f - 1e-6f) / ext : 0.f;
-        any |= ext > 0.f;
+        // (an extent so small that the scale overflows bins nothing on its axis)
+        const float scale = ext > 0.f ? (float)kBins * (1.0f - 1e-6f) / ext : 0.f;
+        bn.scale[a] = std::isfinite(scale) ? scale : 0.f;
+        any |= bn.scale[a] > 0.f;
       }
       int axis = -1, plane = 0;
+      if (n == 2 && any) {
+        // Two items: every plane that separates them gives the same two
+        // children at the same cost, so the binned search below would pick
+        // the first axis on which their bins differ, with the lower bin on
+        // the left.  Same split, without filling and sweeping the bins.
+        for (int a = 0; a < 3 && axis < 0; ++a) {
+          if (bn.scale[a] == 0.f) continue;
+          const int k0 = bn.bin(items[b], a), k1 = bn.bin(items[b + 1], a);
+          if (k0 != k1) {
+            axis = a;
+            if (k0 > k1) std::swap(items[b], items[b + 1]);
+          }
+        }
+        if (axis >= 0) {
+          *lb = Box();
+          *rb = Box();
+          lb->grow(items[b].lo, items[b].hi);
+          rb->grow(items[b + 1].lo, items[b + 1].hi);
+          return b + 1;
+        }
+        any = false;  // no axis separates them: the median split below
+      }
       Box bin_box[3][kBins];
       uint32_t bin_n[3][kBins] = {};
       if (any) {

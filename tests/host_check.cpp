@@ -183,6 +183,14 @@ int main(int argc, char **argv) {
     rd_scene s = make_scene(objs);
     check_scene(s, "degenerate", g, 400);
   }
+  {
+    // centres 1e-39 apart: the binning scale overflows on that axis
+    std::vector<rd_object> objs;
+    for (int i = 0; i < 20; ++i) objs.push_back(object(i % 2, (float)i * 1e-39f, 0.f, 0.f, 0.5f));
+    for (int i = 0; i < 20; ++i) objs.push_back(object(0, 3.f, (float)i * 1e-38f, 1.f, 0.25f));
+    rd_scene s = make_scene(objs);
+    check_scene(s, "denormal spacing", g, 400);
+  }
 
   // radiance ring: live ranges never overlap and stay inside the capacity
   for (uint64_t cap : {7ull, 64ull, 1000ull}) {

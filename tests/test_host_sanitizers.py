@@ -19,7 +19,8 @@ def test_host_code_under_sanitizers(tmp_path):
            os.path.join(ROOT, "raydar_amd", "csrc", "bvh.cpp"),
            os.path.join(ROOT, "raydar_amd", "csrc", "rscn.cpp")]
     flags = ["-std=c++17", "-O1", "-g", "-ffp-contract=off", "-fno-omit-frame-pointer",
-             "-fsanitize=address,undefined", "-fno-sanitize-recover=undefined",
+             "-fsanitize=address,undefined,float-cast-overflow",
+             "-fno-sanitize-recover=undefined,float-cast-overflow",
              "-I", os.path.join(ROOT, "include")]
     build = subprocess.run(["g++", *flags, *src, "-o", exe], capture_output=True, text=True)
     if build.returncode != 0 and "asan" in build.stderr.lower() and "cannot find" in build.stderr:
