@@ -9,6 +9,7 @@ namespace rdh {
 Engine::~Engine() {
   release_buffers();
   for (hipEvent_t e : ev_pool_) (void)hipEventDestroy(e);
+  if (arena_free_) (void)hipEventDestroy(arena_free_);
   if (counters) (void)hipFree(counters);
   if (stream) (void)hipStreamDestroy(stream);
 }
@@ -428,6 +429,18 @@ bool Engine::run_tickets(const DevJob *d_jobs, std::vector<DevTicket> tickets,
       if (!e->init(err)) return false;
       aux_.push_back(std::move(e));
     }
+    // Pipes 1.. write this engine's radiance arena from their own streams:
+    // they start after the work already queued here (the previous pass's
+    // accumulation still reads the arena).
+    if (np > 1) {
+      if (!arena_free_ &&
+          !check(hipEventCreateWithFlags(&arena_free_, hipEventDisableTiming), "hipEventCreate", err))
+        return false;
+      if (!check(hipEventRecord(arena_free_, stream), "hipEventRecord", err)) return false;
+      for (int k = 1; k < np; ++k)
+        if (!check(hipStreamWaitEvent(aux_[k - 1]->stream, arena_free_, 0), "hipStreamWaitEvent", err))
+          return false;
+    }
     struct Pipe {
       Engine *e;
       IterArgs a;
@@ -472,7 +485,8 @@ bool Engine::run_tickets(const DevJob *d_jobs, std::vector<DevTicket> tickets,
       a.L = L;
       a.l_cap = l_alloc;
       a.counters = counters;
-      a.single_job = host_jobs.size() == 1 ? 1 : 0;
+      // the single-job kernels take every path as job 0 / ticket 0
+      a.single_job = host_jobs.size() == 1 && n == 1 ? 1 : 0;
       a.job_cap = (uint32_t)host_jobs.size();
       a.track_done = 0;  // completion is known from the drained queue
       for (const DevJob &j : host_jobs)

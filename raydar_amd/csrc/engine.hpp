@@ -155,6 +155,7 @@ class Engine {
              const IterArgs &a, int parity, uint32_t bound);
   std::vector<hipEvent_t> ev_pool_;
   std::vector<std::unique_ptr<Engine>> aux_;  // pipes 1.. of run_tickets
+  hipEvent_t arena_free_ = nullptr;           // this stream's work before a pass
 };
 
 // Message returned by rd_last_error(NULL) (calls with no renderer handle).

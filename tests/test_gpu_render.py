@@ -308,6 +308,22 @@ def test_pass_divided_over_pipes_matches_oracle(spp, pipes, queue, lds, used):
             assert e.value.status == abi.RD_ERR_INVALID_ARGUMENT
 
 
+def test_passes_over_pipes_share_the_radiance_arena_in_order():
+    # a 65536-slot arena holds 4 samples of 160x90: 12 spp run as three
+    # passes, each divided over four pipes; the pipes of a pass write the
+    # arena slots the previous pass's accumulation reads, so they must start
+    # after it (an event on the main stream)
+    sc = scenegen.plain_1080p(1, 160, 90)
+    with ra.HipRenderer(max_sample_count=12, max_bounces=12, seed=6, pipes=4, queue=4096,
+                        l_capacity=1 << 16) as r:
+        img = r.render_frame(sc)
+        acc = r.accumulator()
+        assert r.get_option("pipes_used") == 4
+    ref = oracle.render(sc, 12, 12, 6)
+    np.testing.assert_array_equal(acc.view(np.uint32), ref.view(np.uint32))
+    np.testing.assert_array_equal(img.reshape(-1), oracle.tonemap(ref, 12))
+
+
 def _host_threads():
     return max(1, min(16, os.cpu_count() or 1))
 
