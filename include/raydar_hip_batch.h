@@ -65,7 +65,7 @@ void rd_batch_destroy(rd_batch *b);
  * engines, each with its own stream, queues and radiance ring; every job is
  * given whole to the one with the fewest pending paths and the host pumps
  * them in turn, so their kernels overlap on the GPU,
- * "solo_paths" (default 48 Mi = 50331648; 0 = off): a job of at least this
+ * "solo_paths" (default 64 Mi = 67108864; 0 = off): a job of at least this
  * many paths (W * H * samples) without a progress image renders on the solo
  * lane, a per-scene engine (the rd_render_frame code path: single-job
  * kernels, small trees in LDS, samples split over pipes) driven by a host

@@ -740,7 +740,7 @@ class StreamerGroup {
 // sets of kernels share the CUs.  A job's image does not depend on which
 // side renders it: each pixel's samples are drawn from the same counters and
 // accumulated in sample order either way.
-constexpr uint64_t kSoloPaths = 48ull << 20;
+constexpr uint64_t kSoloPaths = 64ull << 20;
 
 class SoloLane {
  public:

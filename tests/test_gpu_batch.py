@@ -217,7 +217,7 @@ def test_batch_solo_lane_routing_and_results():
     # image equals the oracle's either way
     specs = _jobs()
     with BatchRenderer() as br:
-        assert br.get_option("solo_paths") == 48 << 20
+        assert br.get_option("solo_paths") == 64 << 20
         with pytest.raises(ra.RaydarError) as e:
             br.set_option("solo_paths", -1)
         assert e.value.status == abi.RD_ERR_INVALID_ARGUMENT
