@@ -170,7 +170,13 @@ hipError_t DeviceCache::alloc(void **p, size_t bytes) {
     }
     ++misses;
   }
-  const hipError_t e = hipMalloc(p, c);
+  hipError_t e = hipMalloc(p, c);
+  if (e == hipErrorOutOfMemory && kept_bytes() > 0) {
+    // the kept buffers may be what fills the device: give them back, retry
+    (void)hipGetLastError();
+    trim();
+    e = hipMalloc(p, c);
+  }
   if (e != hipSuccess) return e;
   std::lock_guard<std::mutex> g(mu_);
   class_of_[*p] = c;

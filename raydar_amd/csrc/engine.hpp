@@ -41,8 +41,9 @@ struct DeviceScene {
 // job's buffers while other pipes have kernels queued would stall the host
 // thread and let those queues run dry.  Buffers are rounded up to size
 // classes (each power of two split in eight steps) and reused by class; they
-// are freed for good by trim(), when more than `keep_max` bytes are kept, and
-// on destruction.  Thread-safe.
+// are freed for good by trim(), when more than `keep_max` bytes are kept, when
+// a new allocation runs out of device memory (then it is retried), and on
+// destruction.  Thread-safe.
 class DeviceCache {
  public:
   ~DeviceCache() { trim(); }
