@@ -73,6 +73,10 @@ struct rd_renderer {
   rdh::DeviceScene scene;
   rdh::DevJob job_h{};
   rdh::DevJob *job_d = nullptr;
+  // pinned copy the job record is uploaded from (a pageable source costs a
+  // staging copy on every upload); uploads only when the record changed
+  rdh::DevJob *job_pin = nullptr;
+  bool job_uploaded = false;
   float *acc = nullptr;
   uchar4 *rgba_d = nullptr;
   uint32_t W = 0, H = 0;
@@ -141,7 +145,9 @@ rd_status snapshot(rd_renderer *r, const rd_scene *s, bool zero_acc) {
   rdh::fill_job(*s, r->scene, r->acc, r->cfg.max_bounces, r->seed, &r->job_h);
   if (!r->job_d && !r->eng.check(hipMalloc(&r->job_d, sizeof(rdh::DevJob)), "hipMalloc(job)", &err))
     return device_fail(r, err);
-  if (!r->eng.check(hipMemcpyAsync(r->job_d, &r->job_h, sizeof(rdh::DevJob),
+  *r->job_pin = r->job_h;  // no upload from it is in flight: every call ends synchronised
+  r->job_uploaded = true;
+  if (!r->eng.check(hipMemcpyAsync(r->job_d, r->job_pin, sizeof(rdh::DevJob),
                                    hipMemcpyHostToDevice, r->eng.stream),
                     "upload job", &err) ||
       !r->eng.check(hipStreamSynchronize(r->eng.stream), "upload job", &err))
@@ -153,9 +159,13 @@ rd_status snapshot(rd_renderer *r, const rd_scene *s, bool zero_acc) {
 // Job fields that can change between passes without a new snapshot.
 rd_status refresh_job(rd_renderer *r) {
   std::string err;
+  if (r->job_uploaded && r->job_h.max_bounces == r->cfg.max_bounces && r->job_h.seed == r->seed)
+    return RD_OK;
   r->job_h.max_bounces = r->cfg.max_bounces;
   r->job_h.seed = r->seed;
-  if (!r->eng.check(hipMemcpyAsync(r->job_d, &r->job_h, sizeof(rdh::DevJob),
+  *r->job_pin = r->job_h;
+  r->job_uploaded = true;
+  if (!r->eng.check(hipMemcpyAsync(r->job_d, r->job_pin, sizeof(rdh::DevJob),
                                    hipMemcpyHostToDevice, r->eng.stream),
                     "upload job", &err))
     return device_fail(r, err);
@@ -260,7 +270,9 @@ rd_status rd_create(const rd_config *config, rd_renderer **out) {
   if (config) r->cfg = *config;
   r->eng.cache = &r->cache;
   std::string err;
-  if (!r->eng.init(&err)) {
+  if (!r->eng.init(&err) ||
+      !r->eng.check(hipHostMalloc((void **)&r->job_pin, sizeof(rdh::DevJob), hipHostMallocDefault),
+                    "hipHostMalloc(job)", &err)) {
     rdh::set_global_error(err);
     delete r;
     return RD_ERR_DEVICE;
@@ -276,6 +288,7 @@ void rd_destroy(rd_renderer *r) {
   r->eng.dfree(r->acc);
   r->eng.dfree(r->rgba_d);
   if (r->job_d) (void)hipFree(r->job_d);
+  if (r->job_pin) (void)hipHostFree(r->job_pin);
   delete r;
 }
 

@@ -10,6 +10,7 @@ Engine::~Engine() {
   release_buffers();
   for (hipEvent_t e : ev_pool_) (void)hipEventDestroy(e);
   if (arena_free_) (void)hipEventDestroy(arena_free_);
+  if (tickets_pin_) (void)hipHostFree(tickets_pin_);
   if (counters) (void)hipFree(counters);
   if (stream) (void)hipStreamDestroy(stream);
 }
@@ -480,12 +481,23 @@ bool Engine::run_tickets(const DevJob *d_jobs, std::vector<DevTicket> tickets,
       uint64_t wk = 0;
       for (const DevTicket &t : tk) wk += t.paths;
       if (!e->ensure_tickets(n, err)) return false;
-      if (!check(hipMemcpyAsync(e->d_tickets, tk.data(), sizeof(DevTicket) * n,
+      // from pinned memory (no staging copy); the previous call's upload
+      // from it has completed, as every pipe was synchronised at its end
+      if (n > e->tickets_pin_cap_) {
+        if (e->tickets_pin_) (void)hipHostFree(e->tickets_pin_);
+        e->tickets_pin_ = nullptr;
+        e->tickets_pin_cap_ = 0;
+        if (!check(hipHostMalloc((void **)&e->tickets_pin_, sizeof(DevTicket) * n, hipHostMallocDefault),
+                   "hipHostMalloc(tickets)", err))
+          return false;
+        e->tickets_pin_cap_ = n;
+      }
+      std::memcpy(e->tickets_pin_, tk.data(), sizeof(DevTicket) * n);
+      if (!check(hipMemcpyAsync(e->d_tickets, e->tickets_pin_, sizeof(DevTicket) * n,
                                 hipMemcpyHostToDevice, e->stream),
                  "upload tickets", err))
         return false;
-      if (!check(hipMemsetAsync(e->d_ticket_done, 0, 8ull * n, e->stream), "hipMemset", err))
-        return false;
+      // (no per-ticket completion counts here: track_done is off below)
       if (!check(launch_set_ctl(e->ctl, wk, n, e->stream), "set ctl", err)) return false;
       IterArgs a = e->iter_args(d_jobs);
       a.L = L;

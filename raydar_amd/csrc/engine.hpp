@@ -157,6 +157,8 @@ class Engine {
   std::vector<hipEvent_t> ev_pool_;
   std::vector<std::unique_ptr<Engine>> aux_;  // pipes 1.. of run_tickets
   hipEvent_t arena_free_ = nullptr;           // this stream's work before a pass
+  DevTicket *tickets_pin_ = nullptr;          // pinned source of run_tickets' uploads
+  uint32_t tickets_pin_cap_ = 0;
 };
 
 // Message returned by rd_last_error(NULL) (calls with no renderer handle).
