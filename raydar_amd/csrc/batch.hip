@@ -125,6 +125,10 @@ class Streamer {
 
   ~Streamer() { teardown(); }
 
+  // A streamer's radiance ring holds 128 Mi slots (2 GB): passes take at
+  // most half of it, and the batch and the server run two or three streamers.
+  Streamer() { eng.opt.l_capacity = 1ull << 27; }
+
   bool init() {
     if (!eng.init(&err)) return false;
     return true;
