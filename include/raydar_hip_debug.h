@@ -71,6 +71,10 @@ rd_status rd_debug_host_trace(const rd_scene *scene, int leaf_size, const float 
  * it does not fit now; ops[i] < 0 releases ticket -ops[i]-1 and writes 0. */
 rd_status rd_debug_ring_ops(uint64_t capacity, const int64_t *ops, uint32_t n, int64_t *out);
 
+/* Host: the size class the reused-buffer cache allocates for a request of
+ * `bytes` (buffers are reused by class): out[i] for bytes[i]. */
+rd_status rd_debug_cache_classes(const uint64_t *bytes, uint32_t n, uint64_t *out);
+
 /* Host: the same tree collapsed to 4-wide nodes.  bvh4_info reports its node
  * count and the stack depth its traversal needs (the kernel keeps
  * stack4 + 3 rows) and checks that every primitive sits in exactly one leaf;

@@ -75,6 +75,7 @@ def lib():
         "rd_debug_host_trace": (C.c_int, [sp, C.c_int, C.c_void_p, C.c_uint32, C.c_void_p,
                                           C.c_void_p]),
         "rd_debug_ring_ops": (C.c_int, [C.c_uint64, C.c_void_p, C.c_uint32, C.c_void_p]),
+        "rd_debug_cache_classes": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p]),
         "rd_debug_host_bvh4_info": (C.c_int, [sp, C.c_int, P(C.c_uint32), P(C.c_uint32),
                                               C.c_char_p, C.c_size_t]),
         "rd_debug_host_trace4": (C.c_int, [sp, C.c_int, C.c_void_p, C.c_uint32, C.c_void_p,
@@ -312,6 +313,14 @@ def ring_ops(capacity, ops):
     ops = np.ascontiguousarray(ops, np.int64)
     out = np.zeros(len(ops), np.int64)
     check(lib().rd_debug_ring_ops(capacity, _ptr(ops), len(ops), _ptr(out)))
+    return out
+
+
+def cache_classes(nbytes):
+    """Size classes of the reused-buffer cache for requests of nbytes."""
+    nbytes = np.ascontiguousarray(nbytes, np.uint64)
+    out = np.zeros(len(nbytes), np.uint64)
+    check(lib().rd_debug_cache_classes(_ptr(nbytes), len(nbytes), _ptr(out)))
     return out
 
 

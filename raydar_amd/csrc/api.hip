@@ -764,6 +764,12 @@ rd_status rd_debug_host_trace(const rd_scene *scene, int leaf, const float *rays
   return RD_OK;
 }
 
+rd_status rd_debug_cache_classes(const uint64_t *bytes, uint32_t n, uint64_t *out) {
+  if (n && (!bytes || !out)) return RD_ERR_INVALID_ARGUMENT;
+  for (uint32_t i = 0; i < n; ++i) out[i] = rdh::DeviceCache::size_class((size_t)bytes[i]);
+  return RD_OK;
+}
+
 rd_status rd_debug_ring_ops(uint64_t capacity, const int64_t *ops, uint32_t n, int64_t *out) {
   if ((n && (!ops || !out)) || capacity == 0) return RD_ERR_INVALID_ARGUMENT;
   rdh::RadianceRing ring(capacity);

@@ -248,3 +248,24 @@ def test_radiance_ring_never_overlaps_live_ranges():
                 t += 1
             else:
                 alive.discard(-op - 1)
+
+
+def test_reused_buffer_size_classes():
+    # a class never rounds a request down, wastes at most an eighth of it
+    # above 4 KB, is its own class (reuse finds it again) and grows with the
+    # request; powers of two are exact
+    rng = np.random.default_rng(3)
+    req = np.concatenate([np.arange(1, 20000, 7), rng.integers(1, 1 << 40, 20000),
+                          (1 << np.arange(12, 41)), (1 << np.arange(12, 41)) + 1]).astype(np.uint64)
+    req.sort()
+    cls = ra.cache_classes(req)
+    assert np.all(cls >= req)
+    assert np.all(cls[req <= 4096] == 4096)
+    big = req > 4096
+    assert np.all(cls[big].astype(np.float64) <= req[big].astype(np.float64) * 1.125)
+    assert np.array_equal(ra.cache_classes(cls), cls)
+    assert np.all(np.diff(cls.astype(np.int64)) >= 0)
+    pw = (1 << np.arange(12, 41)).astype(np.uint64)
+    assert np.array_equal(ra.cache_classes(pw), pw)
+    # eight classes per power of two
+    assert len(np.unique(ra.cache_classes(np.arange(65537, 131073, dtype=np.uint64)))) == 8
