@@ -324,6 +324,26 @@ def test_passes_over_pipes_share_the_radiance_arena_in_order():
     np.testing.assert_array_equal(img.reshape(-1), oracle.tonemap(ref, 12))
 
 
+def test_seed_and_bounce_changes_between_frames_on_one_renderer():
+    # the job record on the device is re-uploaded only when the seed or the
+    # bounce limit changed: every frame must match the oracle for the
+    # settings it was rendered with, also after a scene of another size
+    small = scenegen.plain_small(0)
+    other = scenegen.plain_1080p(1, 96, 54)
+    with ra.HipRenderer(max_sample_count=2, max_bounces=12, seed=1) as r:
+        steps = [(small, 1, 12, None), (small, 2, 12, "seed"), (small, 2, 4, "bounces"),
+                 (small, 2, 4, None), (other, 2, 4, None), (other, 7, 9, "both")]
+        for sc, seed, bounces, change in steps:
+            if change in ("seed", "both"):
+                r.set_seed(seed)
+            if change in ("bounces", "both"):
+                r.max_bounces = bounces
+            img = r.render_frame(sc)
+            ref = oracle.render(sc, 2, bounces, seed)
+            np.testing.assert_array_equal(r.accumulator().view(np.uint32), ref.view(np.uint32))
+            np.testing.assert_array_equal(img.reshape(-1), oracle.tonemap(ref, 2))
+
+
 def _host_threads():
     return max(1, min(16, os.cpu_count() or 1))
 
