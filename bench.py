@@ -250,9 +250,11 @@ def main():
     r.set_samples_per_pass(job.spp)
     r.new_frame(sc)
 
+    frame = np.zeros((sc.height, sc.width, 4), np.uint8)  # the caller's frame buffer, reused
+
     def step():
         r.restart_frame()
-        img = r.render_sample()
+        img = r.render_sample(out=frame)
         assert img is not None and r.sample_count == job.spp
 
     for _ in range(args.warmup):

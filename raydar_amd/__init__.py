@@ -151,14 +151,23 @@ class HipRenderer:
     def new_frame(self, scene):
         check(lib().rd_new_frame(self._h, self._rd(scene)), self._h)
 
-    def render_frame(self, scene):
-        out = np.zeros((scene.height, scene.width, 4), np.uint8)
+    @staticmethod
+    def _image(sc, out):
+        """`out` if given (reused, e.g. a caller's frame buffer), else a new image."""
+        if out is None:
+            return np.zeros((sc.height, sc.width, 4), np.uint8)
+        if out.dtype != np.uint8 or out.size != sc.height * sc.width * 4 or not out.flags.c_contiguous:
+            raise ValueError("out must be a C-contiguous uint8 array of height*width*4 elements")
+        return out
+
+    def render_frame(self, scene, out=None):
+        out = self._image(scene, out)
         check(lib().rd_render_frame(self._h, self._rd(scene), _ptr(out), out.nbytes), self._h)
         return out
 
-    def render_sample(self, scene=None):
+    def render_sample(self, scene=None, out=None):
         sc = scene if scene is not None else self._scene
-        out = np.zeros((sc.height, sc.width, 4), np.uint8)
+        out = self._image(sc, out)
         produced = C.c_int(0)
         check(lib().rd_render_sample(self._h, self._rd(scene), _ptr(out), out.nbytes,
                                      C.byref(produced)), self._h)

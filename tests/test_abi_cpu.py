@@ -269,3 +269,17 @@ def test_reused_buffer_size_classes():
     assert np.array_equal(ra.cache_classes(pw), pw)
     # eight classes per power of two
     assert len(np.unique(ra.cache_classes(np.arange(65537, 131073, dtype=np.uint64)))) == 8
+
+
+def test_output_image_buffer_is_reused_or_rejected():
+    # render_frame / render_sample write into `out` when given (a caller's
+    # frame buffer reused across frames) and refuse buffers of another size,
+    # type or layout before any device work
+    sc = scenegen.plain_small(0)
+    buf = np.zeros((sc.height, sc.width, 4), np.uint8)
+    assert ra.HipRenderer._image(sc, buf) is buf
+    assert ra.HipRenderer._image(sc, None).shape == (sc.height, sc.width, 4)
+    for bad in (np.zeros((sc.height, sc.width, 3), np.uint8), np.zeros((sc.height, sc.width, 4), np.float32),
+                np.zeros((sc.width, sc.height * 8), np.uint8)[:, ::2]):
+        with pytest.raises(ValueError):
+            ra.HipRenderer._image(sc, bad)
