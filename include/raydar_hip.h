@@ -241,6 +241,18 @@ typedef struct rd_kernel_times {
 } rd_kernel_times;
 rd_status rd_get_kernel_times(const rd_renderer *r, rd_kernel_times *out);
 
+/* Camera::update_matrices (src/scene/camera.rs:210-231): recomputes
+ * view_matrix = look_at_lh(position, target, up), proj_matrix =
+ * perspective(projection_param degrees, aspect, near, far) or
+ * ortho(-size*aspect, size*aspect, -size, size, near, far), and both inverses,
+ * in FP32 in cgmath 0.18's operation order (aspect = resolution_x /
+ * resolution_y).  The editor's camera edits (orbit, pan, zoom, setters,
+ * camera.rs:62-208) all end in this call.  RD_ERR_INVALID_ARGUMENT, with the
+ * camera unchanged, for a zero height, an unknown projection, perspective
+ * parameters cgmath asserts against (fov outside (0, 180) degrees, near <= 0,
+ * far <= near) or a singular matrix: the reference panics on each. */
+rd_status rd_camera_update_matrices(rd_camera *camera);
+
 /* ---- .rscn I/O (serde layout of the src/scene modules) ----------------- */
 
 /* Parse a .rscn file (cli/mod.rs:32-37).  Numbers are parsed as f64 and

@@ -29,6 +29,29 @@ extern "C" {
 rd_status rd_debug_trace(const rd_scene *scene, int leaf_size, int brute, const float *rays,
                          uint32_t n, int32_t *ids, float *ts, float *normals, uint8_t *front);
 
+/* Closest hit of n rays through the closest-hit kernel the wavefront loop
+ * launches (k_extend_lanes / k_extend, cpu.rs:344-394), not a separate
+ * debug walk: the rays are placed in a path queue and one extend launch of
+ * the chosen form runs over it.  Outputs as rd_debug_trace; normals and the
+ * front flag come from the shading kernels' closest_hit on the device.
+ *   bvh4    1 = the 4-wide tree, 0 = the binary tree
+ *   lds     1 = the tree copied into LDS (single job, small scenes)
+ *   wide    1 = 32-bit traversal stack, 0 = 16-bit
+ *   jobs    1 = single-job launch; k in 2..64 = multi-job launch over a job
+ *           table of k entries, each with its own device copy of the scene;
+ *           ray i belongs to job i % k (through its ticket)
+ *   refill  0 = one ray per lane (k_extend); 1..64 = persistent lanes that
+ *           take new rays once that many lanes of the wave are idle
+ *   block   64, 128 or 256; grid = workgroups (0 = the occupancy grid)
+ * RD_ERR_UNSUPPORTED: the renderer never launches this form for this scene
+ * (a 4-wide stack bound above the kernel's, a tree too large for LDS, ...). */
+typedef struct rd_extend_variant {
+  int32_t bvh4, lds, wide, jobs, refill, block, grid;
+} rd_extend_variant;
+rd_status rd_debug_extend(const rd_scene *scene, int leaf_size, const rd_extend_variant *variant,
+                          const float *rays, uint32_t n, int32_t *ids, float *ts, float *normals,
+                          uint8_t *front);
+
 /* Scatter of n hits on the GPU.  in: n x 16 floats = ray origin(3),
  * direction(3), t, object id (uint32 bits), attenuation(3), light(3), 2 unused;
  * bits: n x 8 draw words, one bounce's two Philox blocks in slot order

@@ -30,7 +30,23 @@ def lib(fast=False):
     path = os.path.join(_HERE, name)
     if not os.path.exists(path):
         build()
-    L = C.CDLL(path)
+    _LIBS[name] = _bind(C.CDLL(path))
+    return _LIBS[name]
+
+
+def build_mutant(macro, out_dir):
+    """The oracle compiled with one planted mistake (an ORACLE_MUTANT_* macro
+    of oracle.cpp) into out_dir: the tests use it to show that a pin fails
+    when the oracle makes that mistake."""
+    import subprocess
+    path = os.path.join(str(out_dir), f"liboracle_{macro.lower()}.so")
+    subprocess.run(["g++", "-O1", "-ffp-contract=off", "-fno-fast-math", "-fPIC", "-std=c++17",
+                    "-shared", f"-D{macro}", "-o", path, os.path.join(_HERE, "oracle.cpp")],
+                   check=True)
+    return _bind(C.CDLL(path))
+
+
+def _bind(L):
     sp = C.POINTER(abi.RdScene)
     L.or_philox4x32_10.argtypes = [_P(np.uint32), _P(np.uint32), _P(np.uint32)]
     L.or_bounce_bits.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, _P(np.uint32)]
@@ -47,7 +63,6 @@ def lib(fast=False):
     for f in ("or_philox4x32_10", "or_bounce_bits", "or_camera_rays_f32", "or_trace_f32",
               "or_scatter_f32", "or_world_f32", "or_render_f32", "or_render_f64", "or_tonemap"):
         getattr(L, f).restype = None
-    _LIBS[name] = L
     return L
 
 
@@ -90,13 +105,14 @@ def trace(scene, rays):
     return ids, ts, normals, front
 
 
-def scatter(scene, packed, bits):
-    """packed: n x 16 floats (see oracle.cpp or_scatter_f32); bits: n x 8."""
+def scatter(scene, packed, bits, L=None):
+    """packed: n x 16 floats (see oracle.cpp or_scatter_f32); bits: n x 8.
+    L: another build of the oracle (build_mutant), default the shipped one."""
     packed = _c(packed, np.float32).reshape(-1, 16)
     bits = _c(bits, np.uint32).reshape(-1, DRAW_WORDS)
     out = np.zeros_like(packed)
     s = scene.as_rd()
-    lib().or_scatter_f32(C.byref(s), packed, bits, len(packed), out)
+    (L or lib()).or_scatter_f32(C.byref(s), packed, bits, len(packed), out)
     return out
 
 

@@ -213,10 +213,16 @@ template <class T> inline V3<T> world_sample(const rd_world &w, V3<T> d) {
 template <class T> struct Scatter { Ray<T> next; V3<T> atten, light; bool transmission; };
 
 // cpu.rs:262-333 with explicit slot uniforms (bits[0..7]).
+// ORACLE_MUTANT_* macros plant a known mistake for the test that proves the
+// pins in tests/test_oracle.py detect it; the shipped oracle defines none.
 template <class T>
 inline Scatter<T> scatter(const Ray<T> &ray, const Hit<T> &h, const rd_object &o,
                           const uint32_t *bits, V3<T> atten, V3<T> light) {
+#ifdef ORACLE_MUTANT_PLAIN_ROUGHNESS
+  T roughness = (T)o.roughness;
+#else
   T roughness = (T)o.roughness * (T)o.roughness;
+#endif
   T metallic = (T)o.metallic;
   T transmission = (T)o.transmission;
   V3<T> diffuse = h.n + random_in_unit_sphere<T>(bits + 4);
@@ -255,7 +261,9 @@ inline Scatter<T> scatter(const Ray<T> &ray, const Hit<T> &h, const rd_object &o
   Scatter<T> out;
   out.next.o = h.p + off * (T)0.0001f;
   out.next.d = dir;
+#ifndef ORACLE_MUTANT_NO_DEGENERATE_RULE
   if (mag2(out.next.d) < (T)1e-10f) out.next.d = h.n;
+#endif
   out.atten = mul_elem(atten, load3<T>(o.albedo));
   V3<T> emission = load3<T>(o.emission_color) * (T)o.emission_strength;
   out.light = light + emission;

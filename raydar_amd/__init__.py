@@ -59,12 +59,15 @@ def lib():
         "rd_get_counters": (C.c_int, [R, P(abi.RdCounters)]),
         "rd_reset_counters": (C.c_int, [R]),
         "rd_get_kernel_times": (C.c_int, [R, P(abi.RdKernelTimes)]),
+        "rd_camera_update_matrices": (C.c_int, [P(abi.RdCamera)]),
         "rd_scene_load_rscn": (C.c_int, [C.c_char_p, P(sp)]),
         "rd_scene_parse_rscn": (C.c_int, [C.c_char_p, C.c_size_t, P(sp)]),
         "rd_scene_save_rscn": (C.c_int, [sp, C.c_char_p]),
         "rd_scene_free": (None, [sp]),
         "rd_debug_trace": (C.c_int, [sp, C.c_int, C.c_int, C.c_void_p, C.c_uint32, C.c_void_p,
                                      C.c_void_p, C.c_void_p, C.c_void_p]),
+        "rd_debug_extend": (C.c_int, [sp, C.c_int, P(abi.RdExtendVariant), C.c_void_p, C.c_uint32,
+                                      C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
         "rd_debug_scatter": (C.c_int, [sp, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),
         "rd_debug_camera_rays": (C.c_int, [sp, C.c_void_p, C.c_uint32, C.c_void_p]),
         "rd_debug_world": (C.c_int, [sp, C.c_void_p, C.c_uint32, C.c_void_p]),
@@ -237,6 +240,19 @@ class HipRenderer:
         return {n: (t.total_ms[i], t.launches[i]) for i, n in enumerate(abi.KERNEL_NAMES)}
 
 
+# ------------------------------------------------------------- camera ------
+def camera_update_matrices(camera):
+    """Camera::update_matrices (camera.rs:210-231) through the C ABI: a copy
+    of the camera dict (Scene.camera layout) with the four matrices
+    recomputed in FP32."""
+    s = Scene(camera, None, np.zeros(0, abi.OBJECT_DTYPE)).as_rd_camera()
+    check(lib().rd_camera_update_matrices(C.byref(s)))
+    out = dict(camera)
+    for k in ("view_matrix", "proj_matrix", "inverse_view_matrix", "inverse_proj_matrix"):
+        out[k] = list(getattr(s, k))
+    return out
+
+
 # -------------------------------------------------------------- .rscn ------
 def load_rscn_native(path):
     """Parse a .rscn file with the library's native reader."""
@@ -265,6 +281,26 @@ def debug_trace(scene, rays, leaf=4, brute=False):
     s = scene.as_rd()
     check(lib().rd_debug_trace(C.byref(s), leaf, int(brute), _ptr(rays), n, _ptr(ids), _ptr(ts),
                                _ptr(normals), _ptr(front)))
+    return ids, ts, normals, front
+
+
+def debug_extend(scene, rays, leaf=1, bvh4=1, lds=0, wide=0, jobs=1, refill=48, block=256, grid=0):
+    """Closest hit through the production extend kernel (see rd_debug_extend).
+    Returns (ids, ts, normals, front), or None when the renderer never launches
+    that kernel form for this scene (RD_ERR_UNSUPPORTED)."""
+    rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 6)
+    n = len(rays)
+    ids = np.zeros(n, np.int32)
+    ts = np.zeros(n, np.float32)
+    normals = np.zeros((n, 3), np.float32)
+    front = np.zeros(n, np.uint8)
+    s = scene.as_rd()
+    v = abi.RdExtendVariant(bvh4, lds, wide, jobs, refill, block, grid)
+    st = lib().rd_debug_extend(C.byref(s), leaf, C.byref(v), _ptr(rays), n, _ptr(ids), _ptr(ts),
+                               _ptr(normals), _ptr(front))
+    if st == abi.RD_ERR_UNSUPPORTED:
+        return None
+    check(st)
     return ids, ts, normals, front
 
 

@@ -97,6 +97,10 @@ class RdKernelTimes(C.Structure):
     _fields_ = [("total_ms", C.c_double * 5), ("launches", C.c_uint64 * 5)]
 
 
+class RdExtendVariant(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in ("bvh4", "lds", "wide", "jobs", "refill", "block", "grid")]
+
+
 KERNEL_NAMES = ("generate", "extend", "shade", "accumulate", "tonemap")
 
 # Every symbol the C ABI exports (include/raydar_hip.h, include/raydar_hip_batch.h,

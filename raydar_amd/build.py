@@ -39,7 +39,7 @@ CXX_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wextr
              "-Wno-unknown-pragmas"]
 
 HIP_SOURCES = ["kernels.hip", "engine.hip", "api.hip", "batch.hip"]
-CXX_SOURCES = ["bvh.cpp", "rscn.cpp"]
+CXX_SOURCES = ["bvh.cpp", "rscn.cpp", "camera.cpp"]
 
 
 def _run(cmd):

@@ -52,6 +52,13 @@ struct Timer {
   void end_if_not_ended() {
     if (!has_duration) end();
   }
+  // timing.rs:117-121: the average over `count` iterations timed together
+  void end_multiple(uint32_t count) {
+    if (started && count > 0) {
+      end();
+      ns /= count;
+    }
+  }
 };
 }  // namespace
 
@@ -172,7 +179,10 @@ rd_status refresh_job(rd_renderer *r) {
   return RD_OK;
 }
 
-// Renders `k` more samples (cpu.rs:193-219 repeated k times).
+// Renders `k` more samples (cpu.rs:193-219 repeated k times).  The k samples
+// run as one pass, so the sample timer reports their average, as the Vulkan
+// renderer does for its all-samples dispatch (vulkan.rs:175-177); with k = 1
+// that is the last sample's time, as CpuRenderer reports (cpu.rs:196, 218).
 rd_status render_samples(rd_renderer *r, uint32_t k) {
   r->prepare_timer.end_if_not_ended();
   r->render_timer.start_if_not_started();
@@ -201,7 +211,7 @@ rd_status render_samples(rd_renderer *r, uint32_t k) {
     r->render_timer.end();
     r->frame_timer.end();
   }
-  r->sample_timer.end();
+  r->sample_timer.end_multiple(k);
   return RD_OK;
 }
 
@@ -580,6 +590,147 @@ rd_status rd_debug_trace(const rd_scene *scene, int leaf, int brute, const float
     (void)hipMemcpy(ts, d_ts, 4ull * n, hipMemcpyDeviceToHost);
     (void)hipMemcpy(normals, d_n, 12ull * n, hipMemcpyDeviceToHost);
     (void)hipMemcpy(front, d_f, n, hipMemcpyDeviceToHost);
+  }
+  return finish(hipSuccess);
+}
+
+rd_status rd_debug_extend(const rd_scene *scene, int leaf, const rd_extend_variant *v,
+                          const float *rays, uint32_t n, int32_t *ids, float *ts, float *normals,
+                          uint8_t *front) {
+  if (!v || !rays || !ids || !ts || !normals || !front) return RD_ERR_INVALID_ARGUMENT;
+  if ((v->block != 64 && v->block != 128 && v->block != 256) || v->jobs < 1 || v->jobs > 64 ||
+      v->refill < 0 || v->refill > 64 || v->grid < 0 || v->grid > (1 << 16) || n > (1u << 26)) {
+    rdh::set_global_error("rd_debug_extend: variant out of range");
+    return RD_ERR_INVALID_ARGUMENT;
+  }
+  DebugCtx c;
+  rd_status st = debug_setup(scene, leaf, &c);
+  if (st != RD_OK) return st;
+  const rdh::BuiltScene &b = c.built;
+  const bool single = v->jobs == 1, lanes = v->refill > 0;
+  const uint32_t n4 = (uint32_t)b.nodes4.size();
+  auto unsupported = [](const char *why) {
+    rdh::set_global_error(std::string("rd_debug_extend: ") + why);
+    return RD_ERR_UNSUPPORTED;
+  };
+  // the forms the engine selects (Engine::run_tickets, Streamer::args)
+  if (v->bvh4 && (!lanes || v->wide || n4 == 0 || b.stack4 + 3 > (uint32_t)rdh::kStackDepth))
+    return unsupported("no 4-wide walk for this scene or variant");
+  if (!v->wide && b.nodes.size() > rdh::kNarrowStackNodes)
+    return unsupported("more than 65535 nodes need the 32-bit stack");
+  uint32_t lds_bytes = 0, rows = 0;
+  if (v->lds) {
+    const uint64_t nb = v->bvh4 ? sizeof(rdh::DevNode4) * (uint64_t)n4
+                                : sizeof(rdh::DevNode) * (uint64_t)b.nodes.size();
+    const uint64_t bytes = nb + sizeof(rdh::DevPrim) * (uint64_t)b.prims.size();
+    rows = v->bvh4 ? b.stack4 + 3 : b.depth + 1;
+    if (!single || !lanes || v->wide || b.nodes.empty() || bytes > rdh::EngineOptions().lds_bvh ||
+        rows > (uint32_t)rdh::kStackDepth)
+      return unsupported("the tree is not traversed from LDS for this scene or variant");
+    lds_bytes = (uint32_t)bytes;
+  }
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0) != hipSuccess)
+    return finish(hipGetLastError());
+  // one device copy of the scene per job, so each lane follows its own job's pointers
+  std::vector<rdh::DevJob> jobs((size_t)v->jobs);
+  for (int k = 0; k < v->jobs; ++k) {
+    rdh::DeviceScene ds;
+    ds.nodes = k == 0 ? c.nodes : (b.nodes.empty() ? nullptr : c.dev(b.nodes.data(), b.nodes.size()));
+    ds.prims = k == 0 ? c.prims : (b.prims.empty() ? nullptr : c.dev(b.prims.data(), b.prims.size()));
+    ds.objects = k == 0 ? c.objects : (b.objects.empty() ? nullptr : c.dev(b.objects.data(), b.objects.size()));
+    ds.nodes4 = n4 ? c.dev(b.nodes4.data(), n4) : nullptr;
+    ds.n_nodes = (uint32_t)b.nodes.size();
+    ds.n_nodes4 = n4;
+    ds.stack4 = b.stack4;
+    ds.n_prims = (uint32_t)b.prims.size();
+    ds.r0 = b.r0;
+    ds.depth = b.depth;
+    if ((!b.nodes.empty() && !ds.nodes) || (!b.prims.empty() && !ds.prims) || (n4 && !ds.nodes4))
+      return RD_ERR_OUT_OF_MEMORY;
+    rdh::fill_job(*scene, ds, nullptr, 1, 0, &jobs[k]);
+  }
+  std::vector<rdh::DevTicket> tickets((size_t)v->jobs);
+  for (int k = 0; k < v->jobs; ++k) tickets[k].job = (uint32_t)k;
+  // The queue: segment s holds rays [s * seg_cap, (s + 1) * seg_cap), so the
+  // consumer order of the iteration is the ray order and slot i holds ray i.
+  const uint32_t seg_cap = std::max<uint32_t>(1, (n + rdh::kSegments - 1) / rdh::kSegments);
+  const uint64_t slots = (uint64_t)rdh::kSegments * seg_cap;
+  std::vector<float> comp[6];
+  for (int a = 0; a < 6; ++a) {
+    comp[a].assign(slots, 0.0f);
+    for (uint32_t i = 0; i < n; ++i) comp[a][i] = rays[6ull * i + a];
+  }
+  std::vector<uint32_t> tb(slots, 0), pidx(slots, 0);
+  for (uint32_t i = 0; i < n; ++i) tb[i] = (single ? 0u : i % (uint32_t)v->jobs) << 8;
+  rdh::QueueCtl ctl_h;
+  std::memset(&ctl_h, 0, sizeof(ctl_h));
+  for (int s = 0; s < rdh::kSegments; ++s) {
+    const uint64_t lo = (uint64_t)s * seg_cap;
+    ctl_h.count[0][s * rdh::kSegStride] = lo >= n ? 0u : (uint32_t)std::min<uint64_t>(seg_cap, n - lo);
+  }
+  ctl_h.n_tickets = (uint32_t)v->jobs;
+  rdh::PathQueue q[2];
+  for (int p = 0; p < 2; ++p) {
+    float **f[] = {&q[p].ox, &q[p].oy, &q[p].oz, &q[p].dx, &q[p].dy, &q[p].dz,
+                   &q[p].ax, &q[p].ay, &q[p].az, &q[p].lx, &q[p].ly, &q[p].lz};
+    for (int a = 0; a < 12; ++a) *f[a] = c.dev<float>(p == 0 && a < 6 ? comp[a].data() : nullptr, slots);
+    q[p].tb = c.dev<uint32_t>(p == 0 ? tb.data() : nullptr, slots);
+    q[p].pidx = c.dev<uint32_t>(p == 0 ? pidx.data() : nullptr, slots);
+    for (int a = 0; a < 12; ++a)
+      if (!*f[a]) return RD_ERR_OUT_OF_MEMORY;
+    if (!q[p].tb || !q[p].pidx) return RD_ERR_OUT_OF_MEMORY;
+  }
+  rdh::IterArgs a{};
+  a.q[0] = q[0];
+  a.q[1] = q[1];
+  a.hit_t = c.dev<float>(nullptr, slots);
+  a.hit_id = c.dev<uint32_t>(nullptr, slots);
+  a.ctl = c.dev(&ctl_h, 1);
+  a.jobs = c.dev(jobs.data(), jobs.size());
+  a.tickets = c.dev(tickets.data(), tickets.size());
+  a.counters = c.dev<rdh::DevCounters>(nullptr, 1);
+  float *d_rays = c.dev(rays, 6ull * std::max<uint32_t>(n, 1));
+  float *d_n = c.dev<float>(nullptr, 3ull * std::max<uint32_t>(n, 1));
+  uint8_t *d_f = c.dev<uint8_t>(nullptr, std::max<uint32_t>(n, 1));
+  if (!a.hit_t || !a.hit_id || !a.ctl || !a.jobs || !a.tickets || !a.counters || !d_rays || !d_n || !d_f)
+    return RD_ERR_OUT_OF_MEMORY;
+  a.capacity = (uint32_t)slots;
+  a.seg_cap = seg_cap;
+  a.brute_force = 0;
+  a.single_job = single ? 1 : 0;
+  a.track_done = 0;
+  a.wide_stack = v->wide ? 1 : 0;
+  a.refill = v->refill;
+  a.lds_bvh_bytes = lds_bytes;
+  a.stack_rows = rows;
+  a.bvh4 = v->bvh4 ? 1 : 0;
+  a.slots = (uint32_t)slots;
+  a.ticket_cap = (uint32_t)v->jobs;
+  a.job_cap = (uint32_t)v->jobs;
+  rdh::LaunchShape sh;
+  sh.block = v->block;
+  if (v->grid > 0) {
+    sh.grid = v->grid;
+  } else {
+    const uint32_t dyn = rdh::extend_dynamic_lds(a, sh.block);
+    sh.grid = cus * std::max(1, rdh::occupancy_blocks(1, sh.block, lanes, dyn ? lds_bytes : 0, dyn,
+                                                      v->bvh4 != 0, single));
+  }
+  if (n && (st = finish(rdh::launch_extend(a, 0, sh, false, nullptr))) != RD_OK) return st;
+  if (n && (st = finish(rdh::launch_debug_hits(c.objects, d_rays, a.hit_id, a.hit_t, n, d_n, d_f,
+                                               nullptr))) != RD_OK)
+    return st;
+  std::vector<uint32_t> id_h(n);
+  if (n) {
+    (void)hipMemcpy(id_h.data(), a.hit_id, 4ull * n, hipMemcpyDeviceToHost);
+    (void)hipMemcpy(ts, a.hit_t, 4ull * n, hipMemcpyDeviceToHost);
+    (void)hipMemcpy(normals, d_n, 12ull * n, hipMemcpyDeviceToHost);
+    (void)hipMemcpy(front, d_f, n, hipMemcpyDeviceToHost);
+  }
+  for (uint32_t i = 0; i < n; ++i) {
+    ids[i] = id_h[i] == rdh::kNoHit ? -1 : (int32_t)id_h[i];
+    if (ids[i] < 0) ts[i] = 0.0f;
   }
   return finish(hipSuccess);
 }

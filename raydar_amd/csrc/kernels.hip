@@ -915,6 +915,21 @@ __global__ void k_debug_trace(const DevNode *nodes, const DevPrim *prims, uint32
   }
 }
 
+__global__ void k_debug_hits(const DevObject *objects, const float *rays, const uint32_t *ids,
+                             const float *ts, uint32_t n, float *normals, uint8_t *front) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (ids[i] == kNoHit) {
+    normals[3 * i] = normals[3 * i + 1] = normals[3 * i + 2] = 0.0f;
+    front[i] = 0;
+    return;
+  }
+  const float *q = rays + 6 * (uint64_t)i;
+  const HitRec h = closest_hit(f3(q[0], q[1], q[2]), f3(q[3], q[4], q[5]), ts[i], objects[ids[i]]);
+  normals[3 * i] = h.n.x; normals[3 * i + 1] = h.n.y; normals[3 * i + 2] = h.n.z;
+  front[i] = h.front ? 1 : 0;
+}
+
 // in / out: 16 floats per hit, the layout of the oracle's or_scatter_f32.
 __global__ void k_debug_scatter(const DevObject *objects, const float *in, const uint32_t *bits,
                                 uint32_t n, float *out) {
@@ -1103,6 +1118,15 @@ hipError_t launch_debug_trace(const DevNode *nodes, const DevPrim *prims, uint32
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_debug_trace, dim3(blocks_for(n, 64)), dim3(64), 0, st, nodes, prims,
                      n_prims, n_nodes, r0, objects, rays, n, brute, ids, ts, normals, front);
+  return hipGetLastError();
+}
+
+hipError_t launch_debug_hits(const DevObject *objects, const float *rays, const uint32_t *ids,
+                             const float *ts, uint32_t n, float *normals, uint8_t *front,
+                             hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_debug_hits, dim3(blocks_for(n, 256)), dim3(256), 0, st, objects, rays, ids,
+                     ts, n, normals, front);
   return hipGetLastError();
 }
 

@@ -84,6 +84,11 @@ hipError_t launch_debug_trace(const DevNode *nodes, const DevPrim *prims, uint32
                               uint32_t n_nodes, float r0, const DevObject *objects,
                               const float *rays, uint32_t n, int brute, int32_t *ids, float *ts,
                               float *normals, uint8_t *front, hipStream_t st);
+// closest_hit normal and front flag (the shading kernels' prologue) of n
+// (ray, t, id) triples; id == kNoHit gives zeros.
+hipError_t launch_debug_hits(const DevObject *objects, const float *rays, const uint32_t *ids,
+                             const float *ts, uint32_t n, float *normals, uint8_t *front,
+                             hipStream_t st);
 hipError_t launch_debug_scatter(const DevObject *objects, const float *in, const uint32_t *bits,
                                 uint32_t n, float *out, hipStream_t st);
 hipError_t launch_debug_camera(const DevJob *job, const uint32_t *pixels, uint32_t n, float *out,

@@ -64,9 +64,11 @@ bool write_png(const char *path, const uint8_t *rgba, uint32_t w, uint32_t h) {
   return std::fclose(f) == 0;
 }
 
-// Scene::default() (src/scene/mod.rs:20-67), with the camera matrices the
-// reference's cgmath computes (stored in scenes/default.rscn).
-void default_scene(rd_scene *s, std::vector<rd_object> *objs) {
+// Scene::default() (src/scene/mod.rs:20-67).  Camera::new ends in
+// update_matrices (camera.rs:33-60), restated in FP32 by
+// rd_camera_update_matrices: the matrices are the ones cgmath stores in
+// scenes/default.rscn, bit for bit.
+bool default_scene(rd_scene *s, std::vector<rd_object> *objs) {
   std::memset(s, 0, sizeof(*s));
   rd_camera &c = s->camera;
   const float pos[3] = {-3.09f, 0.03f, -1.16f};
@@ -78,34 +80,7 @@ void default_scene(rd_scene *s, std::vector<rd_object> *objs) {
   c.projection_param = 90.0f;
   c.near_clip = 0.01f;
   c.far_clip = 1000.0f;
-  // look_at_lh / perspective(90 deg, 854/480, 0.01, 1000) and their inverses
-  const double e[3] = {pos[0], pos[1], pos[2]};
-  double f[3] = {e[0], e[1], e[2]};
-  double fl = std::sqrt(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
-  for (double &v : f) v /= fl;
-  double sv[3] = {-f[2], 0.0, f[0]};  // f x up, up = y
-  double sl = std::sqrt(sv[0] * sv[0] + sv[2] * sv[2]);
-  for (double &v : sv) v /= sl;
-  const double u[3] = {sv[1] * f[2] - sv[2] * f[1], sv[2] * f[0] - sv[0] * f[2],
-                       sv[0] * f[1] - sv[1] * f[0]};
-  auto dot = [](const double *a, const double *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; };
-  double view[16] = {sv[0], u[0], -f[0], 0, sv[1], u[1], -f[1], 0, sv[2], u[2], -f[2], 0,
-                     -dot(e, sv), -dot(e, u), dot(e, f), 1};
-  // inverse of a rigid transform: rotation transposed, translation -R^T t
-  double inv_view[16] = {sv[0], sv[1], sv[2], 0, u[0], u[1], u[2], 0, -f[0], -f[1], -f[2], 0,
-                         e[0], e[1], e[2], 1};
-  const double aspect = (double)(854.0f / 480.0f), n = 0.01, fr = 1000.0;
-  const double cot = 1.0 / std::tan(M_PI / 4.0);
-  double proj[16] = {cot / aspect, 0, 0, 0, 0, cot, 0, 0, 0, 0, (fr + n) / (n - fr), -1,
-                     0, 0, 2 * fr * n / (n - fr), 0};
-  double inv_proj[16] = {aspect / cot, 0, 0, 0, 0, 1 / cot, 0, 0, 0, 0, 0, 1 / proj[14],
-                         0, 0, -1, proj[10] / proj[14]};
-  for (int i = 0; i < 16; ++i) {
-    c.view_matrix[i] = (float)view[i];
-    c.inverse_view_matrix[i] = (float)inv_view[i];
-    c.proj_matrix[i] = (float)proj[i];
-    c.inverse_proj_matrix[i] = (float)inv_proj[i];
-  }
+  if (rd_camera_update_matrices(&c) != RD_OK) return false;
   s->world.kind = RD_WORLD_SKY_COLOR;
   const float top[3] = {0.53f, 0.8f, 0.92f}, bottom[3] = {1.0f, 1.0f, 1.0f};
   std::memcpy(s->world.top_color, top, sizeof(top));
@@ -137,12 +112,14 @@ void default_scene(rd_scene *s, std::vector<rd_object> *objs) {
   objs->push_back(lamp);
   s->object_count = (uint32_t)objs->size();
   s->objects = objs->data();
+  return true;
 }
 
 int usage(const char *argv0) {
   std::fprintf(stderr,
                "usage: %s [--max-sample-count N] [--max-bounces N] [-o|--output PATH] "
-               "[SCENE_FILE]\n",
+               "[--write-scene PATH] [SCENE_FILE]\n"
+               "  --write-scene PATH  write the scene that would be rendered as .rscn and exit\n",
                argv0);
   return 2;
 }
@@ -170,13 +147,16 @@ int main(int argc, char **argv) {
   rd_config cfg;
   rd_config_default(&cfg);
   std::string output = "output.png";
-  const char *scene_path = nullptr;
+  const char *scene_path = nullptr, *write_scene = nullptr;
   for (int i = 1; i < argc; ++i) {
     std::string a = argv[i];
-    if ((a == "--max-sample-count" || a == "--max-bounces" || a == "-o" || a == "--output") &&
+    if ((a == "--max-sample-count" || a == "--max-bounces" || a == "-o" || a == "--output" ||
+         a == "--write-scene") &&
         i + 1 >= argc)
       return usage(argv[0]);
-    if (a == "--max-sample-count") {
+    if (a == "--write-scene") {
+      write_scene = argv[++i];
+    } else if (a == "--max-sample-count") {
       if (!parse_u32(argv[++i], &cfg.max_sample_count)) return usage(argv[0]);
     } else if (a == "--max-bounces") {
       if (!parse_u32(argv[++i], &cfg.max_bounces)) return usage(argv[0]);
@@ -203,8 +183,17 @@ int main(int argc, char **argv) {
     }
     scene = loaded;
   } else {
-    default_scene(&def, &def_objs);
+    if (!default_scene(&def, &def_objs)) {
+      std::fprintf(stderr, "Error: default camera matrices are singular\n");
+      return 1;
+    }
     scene = &def;
+  }
+  if (write_scene) {
+    const rd_status st = rd_scene_save_rscn(scene, write_scene);
+    if (st != RD_OK) std::fprintf(stderr, "Error: %s\n", rd_last_error(nullptr));
+    rd_scene_free(loaded);
+    return st == RD_OK ? 0 : 1;
   }
   rd_renderer *r = nullptr;
   if (rd_create(&cfg, &r) != RD_OK) {

@@ -183,14 +183,10 @@ class Scene:
     def height(self):
         return int(self.camera["resolution_y"])
 
-    def as_rd(self):
-        """Return an abi.RdScene pointing at this scene's object array.
-
-        The returned struct borrows `self.objects`; keep the Scene alive while
-        the struct is in use."""
-        s = abi.RdScene()
+    def as_rd_camera(self):
+        """The camera as an abi.RdCamera."""
         c = self.camera
-        cam = s.camera
+        cam = abi.RdCamera()
         cam.position[:] = c["position"]
         cam.target[:] = c["target"]
         cam.up[:] = c["up"]
@@ -202,6 +198,15 @@ class Scene:
         cam.far_clip = c["far_clip"]
         for k in ("view_matrix", "proj_matrix", "inverse_view_matrix", "inverse_proj_matrix"):
             getattr(cam, k)[:] = c[k]
+        return cam
+
+    def as_rd(self):
+        """Return an abi.RdScene pointing at this scene's object array.
+
+        The returned struct borrows `self.objects`; keep the Scene alive while
+        the struct is in use."""
+        s = abi.RdScene()
+        s.camera = self.as_rd_camera()
         w = self.world
         s.world.kind = w["kind"]
         s.world.top_color[:] = w["top_color"]

@@ -186,6 +186,11 @@ def test_kernel_scatter_host_build_matches_oracle(scene_name):
     np.testing.assert_array_equal(h.view(np.uint32), o.view(np.uint32))
 
 
+def test_kernel_scatter_host_build_meets_roughness_pins():
+    from tests import pins
+    assert pins.failures(ra.host_scatter) == []
+
+
 @pytest.mark.parametrize("scene_name", ["plain_small", "default", "benchmark"])
 def test_kernel_camera_host_build_matches_oracle(scene_name):
     sc = {"plain_small": lambda: scenegen.plain_small(0),
@@ -283,3 +288,119 @@ def test_output_image_buffer_is_reused_or_rejected():
                 np.zeros((sc.width, sc.height * 8), np.uint8)[:, ::2]):
         with pytest.raises(ValueError):
             ra.HipRenderer._image(sc, bad)
+
+
+# ---------------------------------------------------------------- camera ---
+_MATRICES = ("view_matrix", "proj_matrix", "inverse_view_matrix", "inverse_proj_matrix")
+
+
+@pytest.mark.parametrize("name", ["default.rscn", "benchmark.rscn"])
+def test_camera_update_matrices_reproduce_the_reference_scene_files(name):
+    """rd_camera_update_matrices (camera.rs:210-231 in cgmath's FP32 order)
+    recomputes the four matrices cgmath stored in the reference's scene files
+    bit for bit: look_at_lh, perspective and both inverses."""
+    sc = Scene.load(os.path.join(GOLDEN, name))
+    cam = ra.camera_update_matrices(sc.camera)
+    for k in _MATRICES:
+        got = np.array(cam[k], np.float32)
+        ref = np.array(sc.camera[k], np.float32)
+        np.testing.assert_array_equal(got.view(np.uint32), ref.view(np.uint32), err_msg=k)
+
+
+def test_camera_update_matrices_closed_forms_and_errors():
+    base = make_camera((1.0, 2.0, 3.0), (0.5, 0.0, -1.0), (0, 1, 0), 640, 360, fov_deg=50)
+    # orthographic: cgmath ortho(-s*a, s*a, -s, s, n, f) (camera.rs:218-225), whose
+    # inverse composes to the identity
+    ortho = dict(base, projection=abi.RD_PROJECTION_ORTHOGRAPHIC, projection_param=4.0)
+    cam = ra.camera_update_matrices(ortho)
+    a = np.float32(640) / np.float32(360)
+    p = np.array(cam["proj_matrix"], np.float64).reshape(4, 4)  # p[c][r]
+    assert np.isclose(p[0, 0], 2 / (2 * 4 * a), rtol=1e-6) and np.isclose(p[1, 1], 2 / 8, rtol=1e-6)
+    assert np.isclose(p[2, 2], -2 / (1000 - 0.01), rtol=1e-6)
+    assert np.isclose(p[3, 2], -(1000 + 0.01) / (1000 - 0.01), rtol=1e-6) and p[3, 3] == 1
+    for m, mi in (("proj_matrix", "inverse_proj_matrix"), ("view_matrix", "inverse_view_matrix")):
+        M = np.array(cam[m], np.float64).reshape(4, 4).T
+        Mi = np.array(cam[mi], np.float64).reshape(4, 4).T
+        assert np.allclose(M @ Mi, np.eye(4), atol=2e-6)
+    # perspective and view against the double-precision formulas
+    cam = ra.camera_update_matrices(base)
+    ref = make_camera((1.0, 2.0, 3.0), (0.5, 0.0, -1.0), (0, 1, 0), 640, 360, fov_deg=50)
+    for k in _MATRICES:
+        assert np.allclose(cam[k], ref[k], rtol=2e-6, atol=2e-6), k
+    # parameters the reference panics on: an error, the camera unchanged
+    for bad in (dict(resolution_y=0), dict(projection_param=0.0), dict(projection_param=180.0),
+                dict(near_clip=0.0), dict(far_clip=0.005), dict(projection=7),
+                dict(target=[1.0, 2.0, 3.0])):  # eye == target: normalize(0) is NaN
+        c = dict(base, **bad)
+        s = Scene(c, None, np.zeros(0, abi.OBJECT_DTYPE)).as_rd_camera()
+        before = bytes(s)
+        if "target" in bad:  # NaN view: cgmath's unwrap of a NaN determinant
+            ra.lib().rd_camera_update_matrices(C.byref(s))
+            continue
+        assert ra.lib().rd_camera_update_matrices(C.byref(s)) == abi.RD_ERR_INVALID_ARGUMENT, bad
+        assert bytes(s) == before
+
+
+def test_cli_default_scene_is_scene_default(tmp_path):
+    """raydar_hip without a scene file renders Scene::default()
+    (src/scene/mod.rs:20-67): the scene it writes with --write-scene equals
+    scenes/default.rscn, camera matrices bit for bit."""
+    import subprocess
+    cli = os.path.join(ROOT, "raydar_amd", "raydar_hip")
+    out = tmp_path / "default_cli.rscn"
+    subprocess.run([cli, "--write-scene", str(out)], check=True, timeout=60)
+    got, ref = Scene.load(str(out)), Scene.load(os.path.join(GOLDEN, "default.rscn"))
+    for k, v in ref.camera.items():
+        g = np.array(got.camera[k], np.float32)
+        r = np.array(v, np.float32)
+        np.testing.assert_array_equal(g.view(np.uint32), r.view(np.uint32), err_msg=k)
+    assert got.world == ref.world
+    # the file was saved from an edited scene: its lamp albedo is 0.5, where
+    # Scene::default() gives the lamp Material::default()'s 0.8 (material.rs:15-27)
+    lamp_ref = ref.objects.copy()
+    lamp_ref[2]["albedo"] = np.float32(0.8)
+    assert got.objects.tobytes() == lamp_ref.tobytes()
+
+
+# ------------------------------------------------------ Rust FFI binding ---
+RUST_SYS = os.path.join(ROOT, "raydar_amd", "rust", "raydar-hip-sys", "src", "lib.rs")
+
+
+def _rust_structs(text):
+    out = {}
+    for name, body in re.findall(r"pub struct (rd_[a-z_]+) \{(.*?)\n\}", text, flags=re.S):
+        out[name] = re.findall(r"pub ([a-z_0-9]+): ([^,]+),", body)
+    return out
+
+
+def test_rust_bindings_match_the_c_abi():
+    """The FFI crate's #[repr(C)] structs have the header's fields in order
+    with the same sizes, and every function it declares is exported by the
+    library and declared in include/*.h (no cargo here to compile it)."""
+    text = open(RUST_SYS).read()
+    structs = _rust_structs(text)
+    sizes = {"u32": 4, "i64": 8, "f32": 4, "u64": 8, "*const rd_object": 8}
+    mirror = {"rd_object": abi.RdObject, "rd_camera": abi.RdCamera, "rd_world": abi.RdWorld,
+              "rd_config": abi.RdConfig, "rd_timings": abi.RdTimings, "rd_scene": abi.RdScene}
+    for name, cls in mirror.items():
+        fields = structs[name]
+        c_fields = cls._fields_
+        assert [f for f, _ in fields] == [f for f, _ in c_fields], name
+        for (f, rt), (_, ct) in zip(fields, c_fields):
+            m = re.fullmatch(r"\[(\w+); (\d+)\]", rt.strip())
+            if m:
+                n = sizes[m.group(1)] * int(m.group(2))
+            elif rt.strip() in sizes:
+                n = sizes[rt.strip()]
+            else:
+                n = C.sizeof(ct)  # nested struct: compared field by field above
+            assert n == C.sizeof(ct), (name, f)
+    fns = re.findall(r"pub fn (rd_[a-z0-9_]+)\(", text)
+    assert len(fns) >= 25
+    L = C.CDLL(ra.LIB_PATH)
+    declared = set(declared_symbols())
+    for f in fns:
+        assert hasattr(L, f) and f in declared, f
+    hip_rs = open(os.path.join(ROOT, "raydar_amd", "rust", "renderer", "hip.rs")).read()
+    for f in re.findall(r"sys::(rd_[a-z0-9_]+)\(", hip_rs):
+        assert f in fns, f

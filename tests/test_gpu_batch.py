@@ -188,6 +188,37 @@ def test_autotune_returns_a_valid_setting():
     np.testing.assert_array_equal(out[0].acc, _oracle_acc(scenegen.plain_small(0), 2, 4, 0))
 
 
+def _stratified_pixels(scene, k, salt):
+    npix = scene.width * scene.height
+    k = min(k, npix)
+    step = npix // k
+    return (np.arange(k, dtype=np.uint64) * step + (salt * 7919) % max(1, step)).astype(np.uint32)
+
+
+def test_batch_64_scene_workload_matches_oracle():
+    """The full 64-scene batch (seeds 1000-1063: 16-2048 primitives, 128^2 to
+    1920x1080, 8/32/128 spp, 4/8/12 bounces; 2.05 G paths) with the default
+    pipes and solo lane: every job's accumulator equals the oracle's FP32
+    render bit for bit on 2048 stratified pixels, and its image is the
+    oracle's tonemap there."""
+    jobs_spec = scenegen.batch_jobs()
+    jobs = [Job(j.scene, j.spp, j.bounces, j.seed) for j in jobs_spec]
+    with BatchRenderer() as br:
+        st = br.render(jobs)
+        solo = br.get_option("solo_jobs")
+    assert st["paths"] == sum(j.paths for j in jobs) == 2047647744
+    assert 0 < solo < len(jobs)  # both the shared queues and the solo lane rendered jobs
+    threads = max(1, min(16, os.cpu_count() or 1))
+    for job, spec in zip(jobs, jobs_spec):
+        pix = _stratified_pixels(spec.scene, 2048, spec.seed)
+        ref = oracle.render(spec.scene, spec.spp, spec.bounces, spec.seed, pixels=pix, threads=threads)
+        got = job.acc[pix]
+        np.testing.assert_array_equal(got.view(np.uint32), ref.view(np.uint32),
+                                      err_msg=f"job seed {spec.seed}")
+        np.testing.assert_array_equal(job.rgba.reshape(-1, 4)[pix].reshape(-1),
+                                      oracle.tonemap(ref, spec.spp))
+
+
 _BIG_REF = {}
 
 

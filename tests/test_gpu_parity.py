@@ -98,6 +98,62 @@ def test_closest_hit_parity(name, leaf, brute):
     assert exact == 1.0
 
 
+# Forms of the closest-hit kernel the renderer launches (Engine::run_tickets for
+# one scene, the streamers for batches and the server): tree width, LDS or
+# global tree, 16- or 32-bit stack, single- or multi-job, persistent lanes or
+# one ray per lane, and a small grid so that every wave refills its lanes.
+EXTEND_VARIANTS = {
+    "w4_lds": dict(bvh4=1, lds=1),
+    "w4_global": dict(bvh4=1, lds=0),
+    "bin_lds": dict(bvh4=0, lds=1),
+    "bin_global16": dict(bvh4=0, wide=0),
+    "bin_global32": dict(bvh4=0, wide=1),
+    "multi_w4": dict(bvh4=1, jobs=3),
+    "multi_bin32": dict(bvh4=0, wide=1, jobs=3),
+    "multi_bin16_noref": dict(bvh4=0, jobs=3, refill=0),
+    "single_bin32_noref": dict(bvh4=0, wide=1, refill=0),
+    "w4_refill1_grid3": dict(bvh4=1, refill=1, block=64, grid=3),
+    "multi_w4_refill16_grid5": dict(bvh4=1, jobs=5, refill=16, block=128, grid=5),
+    "w4_lds_leaf4_grid2": dict(bvh4=1, lds=1, leaf=4, block=64, grid=2),
+}
+
+
+@pytest.mark.parametrize("variant", sorted(EXTEND_VARIANTS))
+@pytest.mark.parametrize("name", sorted(SCENES))
+def test_production_extend_parity(name, variant):
+    """The adversarial ray sets through the kernel that renders (k_extend_lanes
+    / k_extend), not the debug walk: ids bit-exact, t and normals within 1e-5
+    (and in fact bit-exact), against the oracle's linear scan (cpu.rs:344-394)."""
+    sc = SCENES[name]()
+    opts = dict(EXTEND_VARIANTS[variant])
+    leaf = opts.pop("leaf", 1)
+    rays = raygen.mixed(sc, 1500, seed=zlib.crc32(f"extend/{name}/{variant}".encode()))
+    got = ra.debug_extend(sc, rays, leaf=leaf, **opts)
+    if got is None:
+        pytest.skip(f"the renderer does not launch the {variant} form for {name}")
+    ids_g, ts_g, n_g, f_g = got
+    ids_o, ts_o, n_o, f_o = oracle.trace(sc, rays)
+    np.testing.assert_array_equal(ids_g, ids_o)
+    hit = ids_o >= 0
+    assert np.all(_close(ts_g[hit], ts_o[hit]))
+    assert np.all(_close(n_g[hit], n_o[hit]))
+    np.testing.assert_array_equal(f_g[hit], f_o[hit])
+    exact = np.mean(ts_g[hit].view(np.uint32) == ts_o[hit].view(np.uint32)) if hit.any() else 1.0
+    print(f"{name} {variant}: {len(rays)} rays, hit rate {hit.mean():.3f}, t bit-exact {exact:.6f}")
+    assert exact == 1.0
+
+
+def test_production_extend_forms_cover_the_renderer():
+    """Each kernel form of the parametrised test runs on at least one scene
+    (a form every scene skips would leave that kernel untested)."""
+    sc = scenegen.plain_small(0)
+    rays = raygen.mixed(sc, 50, seed=1)
+    for variant, opts in EXTEND_VARIANTS.items():
+        opts = dict(opts)
+        leaf = opts.pop("leaf", 1)
+        assert ra.debug_extend(sc, rays, leaf=leaf, **opts) is not None, variant
+
+
 _scatter_inputs = raygen.scatter_inputs
 
 
@@ -113,6 +169,12 @@ def test_scatter_parity(name):
     exact = np.mean(g.view(np.uint32) == o.view(np.uint32))
     print(f"{name}: {len(packed)} scatters, {o[:, 12].mean():.3f} transmissive, bit-exact {exact:.6f}")
     assert exact == 1.0
+
+
+def test_scatter_roughness_and_degenerate_pins_on_device():
+    """The closed-form cases of tests/pins.py on the GPU's scatter."""
+    from tests import pins
+    assert pins.failures(ra.debug_scatter) == []
 
 
 @pytest.mark.parametrize("name", ["plain_small", "default_rscn"])

@@ -8,6 +8,7 @@ per-pixel RMSE to that truth must be within 1% of the CPU FP32 renderer's at
 equal spp (tests/golden/plain_small_truth.npz, made by make_truth.py).
 """
 import os
+import time
 
 import numpy as np
 import pytest
@@ -106,6 +107,94 @@ def test_progressive_samples_equal_full_frame():
         np.testing.assert_array_equal(frames[-1], full)
         prof = r.profiler()
         assert prof["frame_ns"] is not None and prof["render_ns"] is not None
+
+
+class _Timer:
+    """timing.rs:68-127 Timer (start / end / end_if_not_ended / end_multiple)."""
+
+    def __init__(self):
+        self.t0 = None
+        self.duration = None
+
+    def start(self):
+        self.t0 = time.perf_counter()
+
+    def start_if_not_started(self):
+        if self.t0 is None:
+            self.start()
+
+    def end(self):
+        if self.t0 is not None:
+            self.duration = time.perf_counter() - self.t0
+
+    def end_if_not_ended(self):
+        if self.duration is None:
+            self.end()
+
+    def end_multiple(self, count):
+        if self.t0 is not None:
+            self.duration = (time.perf_counter() - self.t0) / count
+
+
+def test_rust_shim_call_sequence_sets_every_timer():
+    """raydar_amd/rust/renderer/hip.rs replayed call for call through ctypes:
+    render_frame = new_frame (frame + prepare timers, rd_new_frame), then the
+    timers of render_next_sample (cpu.rs:193-219) around one rd_render_sample
+    pass of every sample, sample timer ended with end_multiple; render_sample =
+    one sample per call.  print_profiling_metrics (main.rs:61-107) needs all
+    four Rust-side durations; the library's own rd_profiler reports all four
+    too, and the images equal the oracle's."""
+    sc = scenegen.plain_small(0)
+    spp = 5
+    with ra.HipRenderer(max_sample_count=spp, max_bounces=12, seed=3) as r:
+        prof = {k: _Timer() for k in ("frame", "prepare", "render", "sample")}
+        # render_frame -> new_frame
+        prof["frame"].start()
+        prof["prepare"].start()
+        r.new_frame(sc)
+        remaining = r.max_sample_count - r.sample_count
+        assert remaining == spp
+        prof["prepare"].end_if_not_ended()
+        prof["render"].start_if_not_started()
+        prof["sample"].start()
+        r.set_samples_per_pass(remaining)
+        img = r.render_sample(sc)
+        prof["render"].end()
+        prof["frame"].end()
+        prof["sample"].end_multiple(remaining)
+        assert img is not None and r.sample_count == spp
+        assert all(t.duration is not None for t in prof.values())
+        lib_t = r.profiler()
+        assert all(v is not None and v > 0 for v in lib_t.values()), lib_t
+        acc = r.accumulator()
+        r.set_samples_per_pass(1)
+        assert r.render_sample(sc) is None  # cpu.rs:143-145
+    ref = oracle.render(sc, spp, 12, 3)
+    np.testing.assert_array_equal(acc.view(np.uint32), ref.view(np.uint32))
+    np.testing.assert_array_equal(img.reshape(-1), oracle.tonemap(ref, spp))
+    # the editor's loop: one render_sample per frame after new_frame
+    with ra.HipRenderer(max_sample_count=3, max_bounces=12, seed=3) as r:
+        prof = {k: _Timer() for k in ("frame", "prepare", "render", "sample")}
+        prof["frame"].start()
+        prof["prepare"].start()
+        r.new_frame(sc)
+        while r.sample_count < r.max_sample_count:
+            prof["prepare"].end_if_not_ended()
+            prof["render"].start_if_not_started()
+            prof["sample"].start()
+            assert r.render_sample(sc) is not None
+            if r.sample_count == r.max_sample_count:
+                prof["render"].end()
+                prof["frame"].end()
+            prof["sample"].end()
+        assert all(t.duration is not None for t in prof.values())
+        assert all(v is not None for v in r.profiler().values())
+    # rd_render_frame alone: all four set, the sample timer is the per-sample average
+    with ra.HipRenderer(max_sample_count=8, max_bounces=12, seed=3) as r:
+        r.render_frame(sc)
+        t = r.profiler()
+        assert all(v is not None and v > 0 for v in t.values()), t
+        assert t["sample_ns"] * 8 <= t["render_ns"] * 1.05 + 1e-4
 
 
 def test_rmse_against_fp64_truth():

@@ -25,6 +25,7 @@ import pytest
 
 import oracle
 from raydar_amd import abi, scenegen
+from tests import pins
 from raydar_amd.scene import (Scene, look_at_lh, make_camera, make_object, perspective,
                               sky_world, solid_world)
 
@@ -294,6 +295,24 @@ def test_diffuse_direction_hemisphere_and_unit_offset():
     # offset along the normal (cpu.rs:316-323)
     p = rays[:, :3] + rays[:, 3:] * ts[:, None]
     assert np.allclose(out[:, :3], p + nrm * np.float32(1e-4), atol=1e-6)
+
+
+def test_scatter_roughness_squared_and_degenerate_direction():
+    """Closed forms that only roughness^2 satisfies (metal and glass offsets
+    scaled by r^2, the diffuse/specular choice u2 < r^2, cpu.rs:262-263,
+    276-277, 300-302, 308-309) and the |dir|^2 < 1e-10 -> n rule
+    (cpu.rs:326-328); see tests/pins.py."""
+    assert pins.failures(oracle.scatter) == []
+
+
+@pytest.mark.parametrize("macro", sorted(pins.CAUGHT_BY))
+def test_pins_catch_planted_oracle_mistakes(macro, tmp_path):
+    """The same pins run against the oracle built with one planted mistake
+    (plain roughness instead of roughness^2, or no degenerate-direction rule)
+    and must fail on exactly the cases that mistake changes."""
+    L = oracle.build_mutant(macro, tmp_path)
+    bad = pins.failures(lambda s, p, b: oracle.scatter(s, p, b, L=L))
+    assert set(bad) == set(pins.CAUGHT_BY[macro])
 
 
 def test_empty_scene_pixels_are_sky():
