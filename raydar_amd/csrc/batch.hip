@@ -187,6 +187,13 @@ class Streamer {
     j->bounces = d.max_bounces;
     j->seed = d.seed;
     const uint64_t npix = (uint64_t)j->W * j->H;
+    // A pass of one sample must fit the radiance ring, or its ticket could
+    // never be issued and the job would wait forever.
+    if (npix > eng.opt.l_capacity) {
+      *e = "image of " + std::to_string(npix) + " pixels is larger than the radiance ring (" +
+           std::to_string(eng.opt.l_capacity) + " slots)";
+      return nullptr;
+    }
     uint32_t pass = d.pass_samples ? d.pass_samples : (default_pass ? default_pass : d.samples);
     pass = std::min(pass, d.samples);
     const uint64_t fit = std::max<uint64_t>(1, eng.opt.l_capacity / 2 / npix);

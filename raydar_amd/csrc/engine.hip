@@ -147,6 +147,7 @@ bool Engine::ensure_tickets(uint32_t n, std::string *err) {
   if (!check(hipMalloc(&d_tickets, sizeof(DevTicket) * cap), "hipMalloc(tickets)", err)) return false;
   if (!check(hipMalloc(&d_ticket_done, 8ull * cap), "hipMalloc(tickets)", err)) return false;
   ticket_alloc = cap;
+  ++generation;  // a captured graph holds the old table pointers: recapture
   return true;
 }
 
