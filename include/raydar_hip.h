@@ -156,6 +156,28 @@ rd_status rd_render_sample(rd_renderer *r, const rd_scene *scene,
 rd_status rd_render_frame(rd_renderer *r, const rd_scene *scene,
                           uint8_t *rgba, size_t rgba_capacity);
 
+/* rd_render_frame without device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory during the
+ * call (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Same semantics, images, sample count and profiler
+ * timers as rd_render_frame, except that the renderer's resident frame
+ * (rd_read_accumulator, later rd_render_sample passes) is not touched.
+ * Every device buffer of the call (BVH, primitives, accumulator, image,
+ * path queues, radiance slots, control block) is carved from one static
+ * device array of 192 MiB per GPU, set up by rd_create and held until the
+ * process ends; the call itself makes no hipMalloc, hipFree or other
+ * allocating call and uses one stream, with passes of as many samples as
+ * the leftover radiance slots hold.  Limit: 36 bytes per pixel (accumulator,
+ * image, one sample of radiance slots) plus the BVH (about 352 bytes per
+ * object) must fit beside 60 MiB of path queues: 1920x1080 with up to about
+ * 180,000 objects, or up to about 3.8 M pixels without objects; larger
+ * scenes get RD_ERR_INVALID_ARGUMENT with the byte count.  The array keeps the last
+ * call's contents and is shared by every renderer on the GPU: calls from one
+ * process are serialised by a lock, so calls that would overlap on one GPU
+ * wait for each other. */
+rd_status rd_render_frame_noalloc(rd_renderer *r, const rd_scene *scene, uint8_t *rgba,
+                                  size_t rgba_capacity);
+
 /* Renderer::profiler (mod.rs:29). */
 rd_status rd_profiler(const rd_renderer *r, rd_timings *out);
 

@@ -69,6 +69,51 @@ void set_global_error(const std::string &msg) {
 
 }  // namespace rdh
 
+// ------------------------------------------- static scratch (no-alloc path) --
+// Device memory of rd_render_frame_noalloc: one array per GPU, resolved at
+// rd_create and held until the process ends.  Host code must never pass
+// `scratch` itself to a kernel or a copy (it names a host-side stand-in): it
+// uses the device address hipGetSymbolAddress returned for the current GPU.
+namespace rdh {
+constexpr uint64_t kScratchBytes = 192ull << 20;  // 192 MiB per GPU
+constexpr uint32_t kScratchQueue = 1u << 19;      // paths in flight
+constexpr int kScratchGrid = 512;                 // workgroups per launch: seg_cap = queue / 32
+constexpr int kScratchDevices = 64;
+}  // namespace rdh
+
+static __device__ __align__(256) unsigned char scratch[rdh::kScratchBytes];
+
+namespace {
+std::mutex g_scratch_mu;                                  // guards g_scratch_addr
+void *g_scratch_addr[rdh::kScratchDevices] = {};          // `scratch` on each GPU
+std::mutex g_scratch_use[rdh::kScratchDevices];           // one no-alloc call per GPU at a time
+
+// Once per device, with that device current.
+bool scratch_setup(int device, std::string *err) {
+  if (device < 0 || device >= rdh::kScratchDevices) {
+    *err = "device index beyond the static scratch table";
+    return false;
+  }
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (g_scratch_addr[device]) return true;
+  void *p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) {
+    *err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+    (void)hipGetLastError();
+    return false;
+  }
+  g_scratch_addr[device] = p;
+  return true;
+}
+
+void *scratch_addr(int device) {
+  if (device < 0 || device >= rdh::kScratchDevices) return nullptr;
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  return g_scratch_addr[device];
+}
+}  // namespace
+
 struct rd_renderer {
   rd_config cfg{1024, 12};
   uint32_t sample_count = 0;
@@ -77,6 +122,7 @@ struct rd_renderer {
   std::string last_error;
   rdh::DeviceCache cache;  // scene and image buffers reused across frames (outlives eng)
   rdh::Engine eng;
+  rdh::Engine noalloc_eng;  // rd_render_frame_noalloc: borrows the static scratch
   rdh::DeviceScene scene;
   rdh::DevJob job_h{};
   rdh::DevJob *job_d = nullptr;
@@ -282,7 +328,8 @@ rd_status rd_create(const rd_config *config, rd_renderer **out) {
   std::string err;
   if (!r->eng.init(&err) ||
       !r->eng.check(hipHostMalloc((void **)&r->job_pin, sizeof(rdh::DevJob), hipHostMallocDefault),
-                    "hipHostMalloc(job)", &err)) {
+                    "hipHostMalloc(job)", &err) ||
+      !scratch_setup(r->eng.device, &err)) {
     rdh::set_global_error(err);
     delete r;
     return RD_ERR_DEVICE;
@@ -358,6 +405,166 @@ rd_status rd_render_frame(rd_renderer *r, const rd_scene *scene, uint8_t *rgba, 
     if (st != RD_OK) return st;
   }
   return tonemap_out(r, rgba, cap);
+}
+
+rd_status rd_render_frame_noalloc(rd_renderer *r, const rd_scene *scene, uint8_t *rgba,
+                                  size_t cap) {
+  if (!r) return RD_ERR_INVALID_ARGUMENT;
+  std::string err;
+  if (!valid_scene(scene, &err))
+    return r->fail(scene && scene->world.kind == RD_WORLD_TRANSPARENT ? RD_ERR_UNSUPPORTED
+                                                                       : RD_ERR_INVALID_ARGUMENT,
+                   err);
+  const uint32_t W = scene->camera.resolution_x, H = scene->camera.resolution_y;
+  const uint64_t npix = (uint64_t)W * H;
+  if (!rgba || cap < 4 * npix) return r->fail(RD_ERR_BUFFER_TOO_SMALL, "image buffer smaller than W*H*4 bytes");
+  const int dev = r->eng.device;
+  uint8_t *base = static_cast<uint8_t *>(scratch_addr(dev));
+  if (!base) return r->fail(RD_ERR_DEVICE, "static scratch not set up on this device");
+  std::lock_guard<std::mutex> one_at_a_time(g_scratch_use[dev]);
+
+  rdh::BuiltScene b;
+  if (!rdh::build_scene(*scene, r->eng.opt.leaf, &b, &err)) return r->fail(RD_ERR_INVALID_ARGUMENT, err);
+  if (!rdh::check_bvh(b, &err)) return r->fail(RD_ERR_INVALID_ARGUMENT, "BVH build: " + err);
+
+  // The engine's launch shape decides the queue segments: a fixed grid keeps
+  // them at queue / 32 slots for every block size.
+  rdh::Engine &se = r->noalloc_eng;
+  se.opt = r->eng.opt;
+  se.opt.grid = rdh::kScratchGrid;
+  se.opt.queue = rdh::kScratchQueue;
+  se.opt.pipes = 1;
+  se.opt.count = 0;
+  se.opt.kernel_timing = 0;
+  se.num_cus = r->eng.num_cus;
+  const uint64_t seg_cap = se.segment_slots();
+  const uint64_t slots = (uint64_t)rdh::kSegments * seg_cap;
+
+  // Layout of the scratch array, 256-byte aligned pieces.
+  uint64_t off = 0;
+  auto take = [&](uint64_t bytes) {
+    off = (off + 255) & ~255ull;
+    const uint64_t at = off;
+    off += bytes;
+    return at;
+  };
+  const uint64_t o_ctl = take(sizeof(rdh::QueueCtl)), o_cnt = take(sizeof(rdh::DevCounters));
+  const uint64_t o_job = take(sizeof(rdh::DevJob)), o_tk = take(sizeof(rdh::DevTicket));
+  const uint64_t o_done = take(8);
+  const uint64_t o_nodes = take(sizeof(rdh::DevNode) * b.nodes.size());
+  const uint64_t o_nodes4 = take(sizeof(rdh::DevNode4) * b.nodes4.size());
+  const uint64_t o_prims = take(sizeof(rdh::DevPrim) * b.prims.size());
+  const uint64_t o_obj = take(sizeof(rdh::DevObject) * b.objects.size());
+  const uint64_t o_acc = take(16 * npix), o_img = take(4 * npix);
+  uint64_t o_q[2][14];
+  for (int p = 0; p < 2; ++p)
+    for (int a = 0; a < 14; ++a) o_q[p][a] = take(4 * slots);
+  const uint64_t o_hit_t = take(4 * slots), o_hit_id = take(4 * slots);
+  const uint64_t o_L = take(0);
+  // the radiance arena takes the rest: at least one sample of every pixel
+  const uint64_t l_slots = off < rdh::kScratchBytes ? (rdh::kScratchBytes - off) / 16 : 0;
+  if (l_slots < npix)
+    return r->fail(RD_ERR_INVALID_ARGUMENT,
+                   "scene needs " + std::to_string(off + 16 * npix) + " bytes of the " +
+                       std::to_string(rdh::kScratchBytes) +
+                       "-byte static scratch (image, BVH, queues and one sample of radiance slots); "
+                       "rd_render_frame has no such limit");
+
+  rdh::ExternalBuffers eb{};
+  for (int p = 0; p < 2; ++p) {
+    float **f[] = {&eb.q[p].ox, &eb.q[p].oy, &eb.q[p].oz, &eb.q[p].dx, &eb.q[p].dy, &eb.q[p].dz,
+                   &eb.q[p].ax, &eb.q[p].ay, &eb.q[p].az, &eb.q[p].lx, &eb.q[p].ly, &eb.q[p].lz};
+    for (int a = 0; a < 12; ++a) *f[a] = reinterpret_cast<float *>(base + o_q[p][a]);
+    eb.q[p].tb = reinterpret_cast<uint32_t *>(base + o_q[p][12]);
+    eb.q[p].pidx = reinterpret_cast<uint32_t *>(base + o_q[p][13]);
+  }
+  eb.hit_t = reinterpret_cast<float *>(base + o_hit_t);
+  eb.hit_id = reinterpret_cast<uint32_t *>(base + o_hit_id);
+  eb.ctl = reinterpret_cast<rdh::QueueCtl *>(base + o_ctl);
+  eb.L = reinterpret_cast<float4 *>(base + o_L);
+  eb.tickets = reinterpret_cast<rdh::DevTicket *>(base + o_tk);
+  eb.ticket_done = reinterpret_cast<unsigned long long *>(base + o_done);
+  eb.counters = reinterpret_cast<rdh::DevCounters *>(base + o_cnt);
+  eb.queue = rdh::kScratchQueue;
+  eb.seg_cap = (uint32_t)seg_cap;
+  eb.l_slots = l_slots;
+  eb.tickets_n = 1;
+  se.bind_external(eb, r->eng);
+
+  // new_frame (cpu.rs:135-140): timers, then the snapshot into the scratch
+  r->frame_timer.start();
+  r->prepare_timer.start();
+  r->sample_count = 0;
+  rdh::DeviceScene ds;
+  ds.nodes = reinterpret_cast<rdh::DevNode *>(base + o_nodes);
+  ds.nodes4 = reinterpret_cast<rdh::DevNode4 *>(base + o_nodes4);
+  ds.prims = reinterpret_cast<rdh::DevPrim *>(base + o_prims);
+  ds.objects = reinterpret_cast<rdh::DevObject *>(base + o_obj);
+  ds.n_nodes = (uint32_t)b.nodes.size();
+  ds.n_nodes4 = (uint32_t)b.nodes4.size();
+  ds.stack4 = b.stack4;
+  ds.n_prims = (uint32_t)b.prims.size();
+  ds.r0 = b.r0;
+  ds.depth = b.depth;
+  float *acc = reinterpret_cast<float *>(base + o_acc);
+  uchar4 *img = reinterpret_cast<uchar4 *>(base + o_img);
+  rdh::DevJob job;
+  rdh::fill_job(*scene, ds, acc, r->cfg.max_bounces, r->seed, &job);
+  rdh::DevJob *d_job = reinterpret_cast<rdh::DevJob *>(base + o_job);
+  const hipStream_t st = se.stream;
+  struct Up {
+    void *dst;
+    const void *src;
+    uint64_t bytes;
+  } ups[] = {{ds.nodes, b.nodes.data(), sizeof(rdh::DevNode) * b.nodes.size()},
+             {ds.nodes4, b.nodes4.data(), sizeof(rdh::DevNode4) * b.nodes4.size()},
+             {ds.prims, b.prims.data(), sizeof(rdh::DevPrim) * b.prims.size()},
+             {ds.objects, b.objects.data(), sizeof(rdh::DevObject) * b.objects.size()},
+             {d_job, &job, sizeof(job)}};
+  // every call starts from its own state: the accumulator is cleared by a
+  // kernel and the control block by run_tickets; the rest is written before
+  // it is read
+  bool ok = true;
+  for (const Up &u : ups)
+    if (ok && u.bytes)
+      ok = se.check(hipMemcpyAsync(u.dst, u.src, u.bytes, hipMemcpyHostToDevice, st), "upload scene", &err);
+  ok = ok && se.check(rdh::launch_zero_acc(acc, 4 * npix, st), "zero accumulator", &err);
+  // render_next_sample timers (cpu.rs:194-196, 212-218) around passes of
+  // as many samples as the radiance slots hold, in sample order
+  const std::vector<rdh::DevJob> host_jobs{job};
+  const uint32_t total = r->cfg.max_sample_count;
+  if (ok && total > 0) {
+    r->prepare_timer.end_if_not_ended();
+    r->render_timer.start_if_not_started();
+    r->sample_timer.start();
+    const uint64_t fit = std::min<uint64_t>(l_slots / npix, 0xffffffffull / npix);
+    for (uint32_t done = 0; ok && done < total;) {
+      rdh::DevTicket t{};
+      t.job = 0;
+      t.sample_base = done;
+      t.spp = (uint32_t)std::min<uint64_t>(total - done, fit);
+      t.paths = (uint32_t)(t.spp * npix);
+      ok = se.run_tickets(d_job, {t}, host_jobs, &err);
+      done += t.spp;
+    }
+  }
+  const int grid = (int)std::min<uint64_t>((npix + 255) / 256, 8192);
+  ok = ok &&
+       se.check(rdh::launch_tonemap(d_job, (uint32_t)npix, (float)(ok ? total : 0), img, eb.counters,
+                                    false, grid, st),
+                "tonemap", &err) &&
+       se.check(hipMemcpyAsync(rgba, img, 4 * npix, hipMemcpyDeviceToHost, st), "read image", &err);
+  std::string sync_err;
+  const bool synced = se.check(hipStreamSynchronize(st), "render", &sync_err);
+  if (ok && !synced) err = sync_err;
+  if (!ok || !synced) return device_fail(r, err);
+  r->sample_count = total;
+  if (total > 0) {
+    r->render_timer.end();
+    r->frame_timer.end();
+    r->sample_timer.end_multiple(total);
+  }
+  return RD_OK;
 }
 
 rd_status rd_profiler(const rd_renderer *r, rd_timings *out) {

@@ -7,6 +7,7 @@
 namespace rdh {
 
 Engine::~Engine() {
+  if (external_) return;  // nothing here is owned
   release_buffers();
   for (hipEvent_t e : ev_pool_) (void)hipEventDestroy(e);
   if (arena_free_) (void)hipEventDestroy(arena_free_);
@@ -46,7 +47,32 @@ bool Engine::init(std::string *err) {
   return check(hipStreamSynchronize(stream), "hipStreamSynchronize", err);
 }
 
+void Engine::bind_external(const ExternalBuffers &b, const Engine &owner) {
+  external_ = true;
+  device = owner.device;
+  num_cus = owner.num_cus;
+  stream = owner.stream;
+  ctl_host = owner.ctl_host;
+  q[0] = b.q[0];
+  q[1] = b.q[1];
+  hit_t = b.hit_t;
+  hit_id = b.hit_id;
+  ctl = b.ctl;
+  L = b.L;
+  d_tickets = b.tickets;
+  d_ticket_done = b.ticket_done;
+  counters = b.counters;
+  q_alloc = b.queue;
+  seg_cap = b.seg_cap;
+  l_alloc = b.l_slots;
+  ticket_alloc = b.tickets_n;
+  opt.queue = b.queue;
+  opt.l_capacity = b.l_slots;
+  opt.pipes = 1;
+}
+
 void Engine::release_buffers() {
+  if (external_) return;
   for (int p = 0; p < 2; ++p) {
     float **f[] = {&q[p].ox, &q[p].oy, &q[p].oz, &q[p].dx, &q[p].dy, &q[p].dz,
                    &q[p].ax, &q[p].ay, &q[p].az, &q[p].lx, &q[p].ly, &q[p].lz};
@@ -88,6 +114,14 @@ uint32_t Engine::queued(const QueueCtl &c, int parity) {
 bool Engine::ensure_buffers(std::string *err) {
   if (opt.queue < 1024) opt.queue = 1024;
   const uint32_t need = segment_slots();
+  if (external_) {
+    // borrowed buffers: they must already fit these options
+    if (opt.queue != q_alloc || need > seg_cap || opt.l_capacity > l_alloc) {
+      *err = "launch shape needs larger borrowed buffers (internal error)";
+      return false;
+    }
+    return true;
+  }
   if (q_alloc != opt.queue || need > seg_cap) {
     const uint64_t slots = (uint64_t)kSegments * need;
     for (int p = 0; p < 2; ++p) {
@@ -139,6 +173,10 @@ uint64_t Engine::device_bytes() const {
 
 bool Engine::ensure_tickets(uint32_t n, std::string *err) {
   if (n <= ticket_alloc) return true;
+  if (external_) {
+    *err = "more tickets than the borrowed table holds (internal error)";
+    return false;
+  }
   uint32_t cap = std::max<uint32_t>(n, 2 * ticket_alloc);
   if (d_tickets) (void)hipFree(d_tickets);
   if (d_ticket_done) (void)hipFree(d_ticket_done);
@@ -482,22 +520,31 @@ bool Engine::run_tickets(const DevJob *d_jobs, std::vector<DevTicket> tickets,
       uint64_t wk = 0;
       for (const DevTicket &t : tk) wk += t.paths;
       if (!e->ensure_tickets(n, err)) return false;
-      // from pinned memory (no staging copy); the previous call's upload
-      // from it has completed, as every pipe was synchronised at its end
-      if (n > e->tickets_pin_cap_) {
-        if (e->tickets_pin_) (void)hipHostFree(e->tickets_pin_);
-        e->tickets_pin_ = nullptr;
-        e->tickets_pin_cap_ = 0;
-        if (!check(hipHostMalloc((void **)&e->tickets_pin_, sizeof(DevTicket) * n, hipHostMallocDefault),
-                   "hipHostMalloc(tickets)", err))
+      if (e->external_) {
+        // no pinned copy to allocate: the runtime stages the pageable source
+        if (!check(hipMemcpyAsync(e->d_tickets, tk.data(), sizeof(DevTicket) * n,
+                                  hipMemcpyHostToDevice, e->stream),
+                   "upload tickets", err))
           return false;
-        e->tickets_pin_cap_ = n;
+      } else {
+        // from pinned memory (no staging copy); the previous call's upload
+        // from it has completed, as every pipe was synchronised at its end
+        if (n > e->tickets_pin_cap_) {
+          if (e->tickets_pin_) (void)hipHostFree(e->tickets_pin_);
+          e->tickets_pin_ = nullptr;
+          e->tickets_pin_cap_ = 0;
+          if (!check(hipHostMalloc((void **)&e->tickets_pin_, sizeof(DevTicket) * n,
+                                   hipHostMallocDefault),
+                     "hipHostMalloc(tickets)", err))
+            return false;
+          e->tickets_pin_cap_ = n;
+        }
+        std::memcpy(e->tickets_pin_, tk.data(), sizeof(DevTicket) * n);
+        if (!check(hipMemcpyAsync(e->d_tickets, e->tickets_pin_, sizeof(DevTicket) * n,
+                                  hipMemcpyHostToDevice, e->stream),
+                   "upload tickets", err))
+          return false;
       }
-      std::memcpy(e->tickets_pin_, tk.data(), sizeof(DevTicket) * n);
-      if (!check(hipMemcpyAsync(e->d_tickets, e->tickets_pin_, sizeof(DevTicket) * n,
-                                hipMemcpyHostToDevice, e->stream),
-                 "upload tickets", err))
-        return false;
       // (no per-ticket completion counts here: track_done is off below)
       if (!check(launch_set_ctl(e->ctl, wk, n, e->stream), "set ctl", err)) return false;
       IterArgs a = e->iter_args(d_jobs);

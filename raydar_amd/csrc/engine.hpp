@@ -83,11 +83,33 @@ struct EngineOptions {
                                   // LDS-resident scenes, else 3)
 };
 
+// Device buffers an engine borrows instead of allocating (the allocation-free
+// entry point carves them from a static device array).
+struct ExternalBuffers {
+  PathQueue q[2];
+  float *hit_t;
+  uint32_t *hit_id;
+  QueueCtl *ctl;
+  float4 *L;
+  DevTicket *tickets;
+  unsigned long long *ticket_done;
+  DevCounters *counters;
+  uint32_t queue;       // nominal paths in flight
+  uint32_t seg_cap;     // slots per queue segment
+  uint64_t l_slots;     // radiance slots
+  uint32_t tickets_n;   // ticket table entries
+};
+
 class Engine {
  public:
   ~Engine();
   // Selects the device and creates the stream; allocation is lazy.
   bool init(std::string *err);
+  // Borrows every device buffer from `b`, the stream and pinned control-block
+  // copy from `owner` (an initialised engine): no allocation, no pipes.  The
+  // engine frees nothing it borrowed.
+  void bind_external(const ExternalBuffers &b, const Engine &owner);
+  bool external() const { return external_; }
   bool ensure_buffers(std::string *err);  // (re)allocates for current options
   void release_buffers();
 
@@ -155,6 +177,7 @@ class Engine {
                                                              const LaunchShape &, bool,
                                                              hipStream_t),
              const IterArgs &a, int parity, uint32_t bound);
+  bool external_ = false;  // buffers, stream and control copy are borrowed
   std::vector<hipEvent_t> ev_pool_;
   std::vector<std::unique_ptr<Engine>> aux_;  // pipes 1.. of run_tickets
   hipEvent_t arena_free_ = nullptr;           // this stream's work before a pass

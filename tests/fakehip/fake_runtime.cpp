@@ -363,6 +363,13 @@ hipError_t hipFree(void *p) {
   return hipSuccess;
 }
 
+hipError_t hipGetSymbolAddress(void **p, const void *symbol, size_t bytes) {
+  std::lock_guard<std::mutex> g(g_mu);
+  g_dev.emplace((uintptr_t)symbol, Alloc{bytes, true, {}});  // static: never freed
+  *p = const_cast<void *>(symbol);
+  return hipSuccess;
+}
+
 hipError_t hipHostMalloc(void **p, size_t bytes, unsigned) {
   *p = std::malloc(bytes ? bytes : 1);
   if (!*p) return ret(hipErrorOutOfMemory);

@@ -20,6 +20,7 @@
 #define __global__
 #define __shared__
 #define __forceinline__ inline
+#define __align__(n) __attribute__((aligned(n)))
 
 typedef enum hipError_t {
   hipSuccess = 0,
@@ -121,6 +122,11 @@ hipError_t hipEventDestroy(hipEvent_t e);
 hipError_t hipEventRecord(hipEvent_t e, hipStream_t s);
 hipError_t hipEventSynchronize(hipEvent_t e);
 hipError_t hipEventElapsedTime(float *ms, hipEvent_t a, hipEvent_t b);
+
+// A __device__ variable is a host array here; HIP_SYMBOL also passes its
+// size so the fake can register it as device memory.
+#define HIP_SYMBOL(X) (&(X)), sizeof(X)
+hipError_t hipGetSymbolAddress(void **p, const void *symbol, size_t bytes);
 
 hipError_t hipGraphInstantiate(hipGraphExec_t *x, hipGraph_t g, hipGraphNode_t *err_node,
                                char *log, size_t log_size);

@@ -383,6 +383,38 @@ void server(double seconds, double speed, uint64_t budget) {
   if (st.jobs_done != arr.size()) fail("server finished " + std::to_string(st.jobs_done) + " jobs");
 }
 
+// rd_render_frame_noalloc: every device buffer from the static array; no
+// allocation or free on the fake device during a call; the same marks as
+// rd_render_frame; a scene beyond the array is refused.
+void noalloc() {
+  auto jobs = batch_jobs(1000, 16);
+  rd_config cfg{1, 12};
+  rd_renderer *r = nullptr;
+  if (rd_create(&cfg, &r) != RD_OK) return fail("rd_create");
+  for (const JobSpec &j : jobs) {
+    rd_set_max_sample_count(r, j.spp);
+    rd_set_max_bounces(r, j.bounces);
+    rd_set_seed(r, j.seed);
+    Out o;
+    o.rgba.assign(4ull * j.scene->s.camera.resolution_x * j.scene->s.camera.resolution_y, 0);
+    const auto before = fakehip::stats();
+    if (rd_render_frame_noalloc(r, &j.scene->s, o.rgba.data(), o.rgba.size()) != RD_OK) {
+      fail(std::string("render_frame_noalloc: ") + rd_last_error(r));
+      continue;
+    }
+    const auto after = fakehip::stats();
+    if (after.mallocs != before.mallocs || after.frees != before.frees)
+      fail("render_frame_noalloc allocated or freed device memory");
+    if (rd_sample_count(r) != j.spp) fail("render_frame_noalloc sample count");
+    check_out(j, o, "noalloc");
+  }
+  auto huge = clustered(9, 50, 4096, 4096);
+  std::vector<uint8_t> img(4ull * 4096 * 4096);
+  if (rd_render_frame_noalloc(r, &huge->s, img.data(), img.size()) != RD_ERR_INVALID_ARGUMENT)
+    fail("render_frame_noalloc accepted a scene beyond its static array");
+  rd_destroy(r);
+}
+
 void sequential_plain() {
   auto jobs = batch_jobs();
   rd_config cfg{1, 12};
@@ -424,6 +456,7 @@ int main(int argc, char **argv) {
     device_clean(name);
   };
   if (scenario == "plain" || scenario == "all") phase("plain", [] { plain(3); });
+  if (scenario == "noalloc" || scenario == "all") phase("noalloc", [] { noalloc(); });
   if (scenario == "batch" || scenario == "all") phase("batch", [] { batch(); });
   if (scenario == "autotune" || scenario == "all") phase("autotune", [] { autotune(); });
   if (scenario == "server" || scenario == "all")

@@ -55,6 +55,7 @@ def _run(exe, args, timeout, **env):
 
 @pytest.mark.parametrize("args,nosync", [
     (["plain"], "0"),
+    (["noalloc"], "1"),
     (["batch"], "1"),
     (["autotune"], "0"),
     (["server", "--seconds", "60", "--speed", "20"], "1"),
