@@ -436,6 +436,7 @@ void k_extend_lanes(IterArgs a, int p) {
   float bt = 0.0f, tcull = INFINITY;
   RayPre r;
   SlabRay s;
+  SlabNF nf;  // the 4-wide walk's near/far form of s
   TraceCounts tc;
   uint32_t rays = 0;
   for (;;) {
@@ -487,6 +488,7 @@ void k_extend_lanes(IterArgs a, int p) {
             prims = J.prims;
           }
           s = slab_ray(r, ray_tolerance(r, SINGLE ? j0_r0 : J.r0_tol));
+          if (W4) nf = slab_nf(s);
           node = 0;
           sp = 0;
           tcull = INFINITY;  // cull_limit of "no hit yet"
@@ -503,22 +505,32 @@ void k_extend_lanes(IterArgs a, int p) {
 #ifdef RDH_CHECKED
         node = (int32_t)chk((uint32_t)node, cur_nodes, K_NODE, a.ctl);
 #endif
-        VecF4 lx, hx, ly, hy, lz, hz;
+        // each axis's near and far planes of the four children, picked by
+        // the ray's direction signs (slab_hit_nf)
+        VecF4 xn, xf, yn, yf, zn, zf;
         VecI4 ch;
         if (LDS) {
-          const VecF4 *np = lds_nodes + 8 * node;
-          lx = np[0]; hx = np[1]; ly = np[2]; hy = np[3]; lz = np[4]; hz = np[5];
-          ch = __builtin_bit_cast(VecI4, np[6]);
+          const unsigned char *np = reinterpret_cast<const unsigned char *>(lds_nodes + 8 * node);
+          xn = *(const VecF4 *)(np + nf.ox); xf = *(const VecF4 *)((np - nf.ox) + 16);
+          yn = *(const VecF4 *)(np + nf.oy); yf = *(const VecF4 *)((np - nf.oy) + 80);
+          zn = *(const VecF4 *)(np + nf.oz); zf = *(const VecF4 *)((np - nf.oz) + 144);
+          ch = *(const VecI4 *)(np + 96);
         } else {
-          GlobalF4 *np = (GlobalF4 *)(nodes4 + node);
-          lx = np[0]; hx = np[1]; ly = np[2]; hy = np[3]; lz = np[4]; hz = np[5];
-          ch = ((GlobalI4 *)np)[6];
+          // 32-bit offsets from the tree's base keep the address arithmetic
+          // (and, for one job, the base itself) out of 64-bit registers
+          const __attribute__((address_space(1))) unsigned char *gb =
+              (const __attribute__((address_space(1))) unsigned char *)nodes4;
+          const uint32_t nb = (uint32_t)node * (uint32_t)sizeof(DevNode4);
+          xn = *(GlobalF4 *)(gb + (nb + nf.ox)); xf = *(GlobalF4 *)(gb + ((nb - nf.ox) + 16u));
+          yn = *(GlobalF4 *)(gb + (nb + nf.oy)); yf = *(GlobalF4 *)(gb + ((nb - nf.oy) + 80u));
+          zn = *(GlobalF4 *)(gb + (nb + nf.oz)); zf = *(GlobalF4 *)(gb + ((nb - nf.oz) + 144u));
+          ch = *(GlobalI4 *)(gb + (nb + 96u));
         }
         float t0, t1, t2, t3;
-        const bool h0 = slab_hit(s, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, tcull, &t0);
-        const bool h1 = slab_hit(s, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, tcull, &t1);
-        const bool h2 = slab_hit(s, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, tcull, &t2);
-        const bool h3 = slab_hit(s, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, tcull, &t3);
+        const bool h0 = slab_hit_nf(nf, xn.x, xf.x, yn.x, yf.x, zn.x, zf.x, tcull, &t0);
+        const bool h1 = slab_hit_nf(nf, xn.y, xf.y, yn.y, yf.y, zn.y, zf.y, tcull, &t1);
+        const bool h2 = slab_hit_nf(nf, xn.z, xf.z, yn.z, yf.z, zn.z, zf.z, tcull, &t2);
+        const bool h3 = slab_hit_nf(nf, xn.w, xf.w, yn.w, yf.w, zn.w, zf.w, tcull, &t3);
         if (COUNT) {  // empty slots (leaf code with no primitives) are not boxes of the tree
           const int32_t e = leaf_code(0, 0);
           tc.box += (ch.x != e) + (ch.y != e) + (ch.z != e) + (ch.w != e);
@@ -529,30 +541,11 @@ void k_extend_lanes(IterArgs a, int p) {
         const uint32_t p1 = n0 + (l1 ? leaf_count(ch.y) : 0u);
         const uint32_t p2 = p1 + (l2 ? leaf_count(ch.z) : 0u);
         const uint32_t n_leaf = p2 + (l3 ? leaf_count(ch.w) : 0u);
-        if (n_leaf) {
-          const uint32_t f0 = leaf_first(ch.x), f1 = leaf_first(ch.y) - n0;
-          const uint32_t f2 = leaf_first(ch.z) - p1, f3 = leaf_first(ch.w) - p2;
-          for (uint32_t t = 0; t < n_leaf; ++t) {
-            DevPrim pr;
-            uint32_t j = t < n0 ? f0 + t : t < p1 ? f1 + t : t < p2 ? f2 + t : f3 + t;
-#ifdef RDH_CHECKED
-            j = chk(j, cur_prims, K_PRIM, a.ctl);
-#endif
-            if (LDS) {
-              const VecF4 pa = lds_prims[2 * j], pb = lds_prims[2 * j + 1];
-              pr.a[0] = pa.x; pr.a[1] = pa.y; pr.a[2] = pa.z; pr.a[3] = pa.w;
-              pr.b[0] = pb.x; pr.b[1] = pb.y; pr.b[2] = pb.z; pr.b[3] = pb.w;
-            } else {
-              load_prim(prims, j, &pr);
-            }
-            if (COUNT) {
-              if (fbits(pr.b[3]) & kTagCube) tc.cube++;
-              else tc.sphere++;
-            }
-            test_prim(r, pr, &bt, &bid);
-          }
-          tcull = cull_limit(bt, bid);
-        }
+        const uint32_t f0 = leaf_first(ch.x), f1 = leaf_first(ch.y) - n0;
+        const uint32_t f2 = leaf_first(ch.z) - p1, f3 = leaf_first(ch.w) - p2;
+        // The next node and the stack are settled before the primitive tests
+        // (they depend only on this node's boxes), so the children's keys
+        // and ids are dead while the primitives are tested.
         // entered inner children in order of entry distance
         const bool i0 = h0 && !l0, i1 = h1 && !l1, i2 = h2 && !l2, i3 = h3 && !l3;
         const int cnt = (int)i0 + (int)i1 + (int)i2 + (int)i3;
@@ -581,13 +574,37 @@ void k_extend_lanes(IterArgs a, int p) {
         }
         const int top = sp > 0 ? sp - 1 : 0;
         const int32_t popped = (int32_t)stk[top * BLOCK];
+        const bool done = cnt == 0 && sp == 0;
         if (cnt > 0) {
           node = c0;
           sp += cnt - 1;
         } else if (sp > 0) {
           node = popped;
           sp = top;
-        } else {
+        }
+        if (n_leaf) {
+          for (uint32_t t = 0; t < n_leaf; ++t) {
+            DevPrim pr;
+            uint32_t j = t < n0 ? f0 + t : t < p1 ? f1 + t : t < p2 ? f2 + t : f3 + t;
+#ifdef RDH_CHECKED
+            j = chk(j, cur_prims, K_PRIM, a.ctl);
+#endif
+            if (LDS) {
+              const VecF4 pa = lds_prims[2 * j], pb = lds_prims[2 * j + 1];
+              pr.a[0] = pa.x; pr.a[1] = pa.y; pr.a[2] = pa.z; pr.a[3] = pa.w;
+              pr.b[0] = pb.x; pr.b[1] = pb.y; pr.b[2] = pb.z; pr.b[3] = pb.w;
+            } else {
+              load_prim(prims, j, &pr);
+            }
+            if (COUNT) {
+              if (fbits(pr.b[3]) & kTagCube) tc.cube++;
+              else tc.sphere++;
+            }
+            test_prim(r, pr, &bt, &bid);
+          }
+          tcull = cull_limit(bt, bid);
+        }
+        if (done) {
           a.hit_t[k] = bid == kNoHit ? 0.0f : bt;
           a.hit_id[k] = bid;
           active = false;

@@ -170,11 +170,9 @@ RD_HD bool hit_sphere(const RayPre &r, const float *pa, const float *pb, float *
   return false;
 }
 
-// cpu.rs:64-98 hit_cube (true division, IEEE minNum/maxNum).  p.a = min, p.b = max.
-RD_HD bool hit_cube(const RayPre &r, const float *pa, const float *pb, float *t) {
-  const float t1x = (pa[0] - r.ox) / r.dx, t2x = (pb[0] - r.ox) / r.dx;
-  const float t1y = (pa[1] - r.oy) / r.dy, t2y = (pb[1] - r.oy) / r.dy;
-  const float t1z = (pa[2] - r.oz) / r.dz, t2z = (pb[2] - r.oz) / r.dz;
+// cpu.rs:64-98 hit_cube from its six slab quotients (IEEE minNum/maxNum).
+RD_HD bool cube_from_quotients(float t1x, float t2x, float t1y, float t2y, float t1z, float t2z,
+                               float *t) {
   const float tmin = fmaxf(fmaxf(fminf(t1x, t2x), fminf(t1y, t2y)), fminf(t1z, t2z));
   const float tmax = fminf(fminf(fmaxf(t1x, t2x), fmaxf(t1y, t2y)), fmaxf(t1z, t2z));
   if (tmax < 0.0f) return false;
@@ -183,15 +181,21 @@ RD_HD bool hit_cube(const RayPre &r, const float *pa, const float *pb, float *t)
   return true;
 }
 
+// cpu.rs:64-98 hit_cube (true division).  p.a = min, p.b = max.
+RD_HD bool hit_cube(const RayPre &r, const float *pa, const float *pb, float *t) {
+  return cube_from_quotients((pa[0] - r.ox) / r.dx, (pb[0] - r.ox) / r.dx, (pa[1] - r.oy) / r.dy,
+                             (pb[1] - r.oy) / r.dy, (pa[2] - r.oz) / r.dz, (pb[2] - r.oz) / r.dz,
+                             t);
+}
+
 // min_by_key(OrderedFloat(t)) over the objects in index order (cpu.rs:345-349):
 // the smallest t wins, -0 == +0, NaN sorts above every number, and among
-// equal keys the lowest object index wins.
+// equal keys the lowest object index wins.  Written without branches: a
+// number beats NaN, a smaller number wins, and equal keys (two equal
+// numbers, or two NaNs) go to the lower index.
 RD_HD bool better_hit(float t, uint32_t id, float bt, uint32_t bid) {
-  if (bid == kNoHit) return true;
   const bool tn = isnan(t), bn = isnan(bt);
-  if (tn != bn) return bn;
-  if (!tn && t != bt) return t < bt;
-  return id < bid;
+  return (bid == kNoHit) | (bn & !tn) | (t < bt) | (((tn & bn) | (t == bt)) & (id < bid));
 }
 
 // Test one primitive and fold it into (best_t, best_id).
@@ -245,6 +249,46 @@ RD_HD bool slab_hit(const SlabRay &s, float lox, float hix, float loy, float hiy
   const float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fmaxf(z0, z1));
   *tenter = tn;
   return tn <= tf && tn <= tcull;
+}
+
+// The same cull with each axis's planes told apart by the direction's sign.
+// For ix > 0, lo <= hi and a >= b (o + tol >= o - tol, scaled by ix) give
+// lo*ix - a <= hi*ix - b exactly, and rounding keeps the order, so
+// min(x0, x1) = fma(lo, ix, -a) and max(x0, x1) = fma(hi, ix, -b); for
+// ix < 0 the roles swap.  slab_hit_nf therefore returns slab_hit's entry
+// distance and verdict for every box with lo <= hi, without the per-axis
+// min/max.  (An empty slot, lo = +inf and hi = -inf, misses here; slab_hit
+// "enters" it as a leaf of zero primitives.  Either way nothing is tested.)
+// The ray keeps, per axis, the byte offset of the near plane's float4 in a
+// DevNode4 (lo_x at 0, hi_x at 16, lo_y at 32, ...; the far plane sits at
+// 16 - ox, 80 - oy, 144 - oz) and the subtrahends in near/far order.
+struct SlabNF {
+  float ix, iy, iz;
+  float nx, ny, nz;     // subtrahend of the near plane: a if ix > 0, else b
+  float fx, fy, fz;     // of the far plane
+  uint32_t ox, oy, oz;  // byte offset of the near plane in a DevNode4
+};
+RD_HD SlabNF slab_nf(const SlabRay &s) {
+  SlabNF n;
+  n.ix = s.ix; n.iy = s.iy; n.iz = s.iz;
+  const bool sx = s.ix < 0.0f, sy = s.iy < 0.0f, sz = s.iz < 0.0f;
+  n.nx = sx ? s.bx : s.ax; n.fx = sx ? s.ax : s.bx;
+  n.ny = sy ? s.by : s.ay; n.fy = sy ? s.ay : s.by;
+  n.nz = sz ? s.bz : s.az; n.fz = sz ? s.az : s.bz;
+  n.ox = sx ? 16u : 0u;
+  n.oy = sy ? 48u : 32u;
+  n.oz = sz ? 80u : 64u;
+  return n;
+}
+RD_HD bool slab_hit_nf(const SlabNF &s, float xn, float xf, float yn, float yf, float zn, float zf,
+                       float tcull, float *tenter) {
+  const float x0 = fmaf(xn, s.ix, -s.nx), x1 = fmaf(xf, s.ix, -s.fx);
+  const float y0 = fmaf(yn, s.iy, -s.ny), y1 = fmaf(yf, s.iy, -s.fy);
+  const float z0 = fmaf(zn, s.iz, -s.nz), z1 = fmaf(zf, s.iz, -s.fz);
+  const float tn = fmaxf(fmaxf(x0, y0), fmaxf(z0, 0.0f));
+  const float tf = fminf(fminf(x1, y1), fminf(z1, tcull));
+  *tenter = tn;
+  return tn <= tf;
 }
 
 // Per-ray dynamic margin: zero while |o|_1 stays inside the radius the static
