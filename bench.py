@@ -274,6 +274,25 @@ def _cpu_model():
     return f"{name} ({', '.join(extra)})" if extra else name
 
 
+def extend_simd(c):
+    """SIMD efficiency of the closest-hit kernel from the counting pass: the
+    share of a wave's 64 lanes that hold a ray per traversal iteration, and
+    per wave iteration the share that orders entered children, the primitive
+    loop's trips and the share of those trips that test a cube."""
+    ws = max(c["wave_steps"], 1)
+    trips = max(c["prim_trips"], 1)
+    return {
+        "lane_utilization": round(c["lane_steps"] / (64.0 * ws), 4),
+        "sort_share": round(c["sort_steps"] / ws, 4),
+        "prim_trips_per_step": round(c["prim_trips"] / ws, 4),
+        "prim_lanes_per_trip": round((c["sphere_tests"] + c["cube_tests"]) / trips, 2),
+        "cube_trip_share": round(c["cube_trips"] / trips, 4),
+        "box_tests_per_lane_step": round(c["box_tests"] / max(c["lane_steps"], 1), 3),
+        "steps_per_refill": round(ws / max(c["refills"], 1), 2),
+        "wave_steps": c["wave_steps"],
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -407,6 +426,7 @@ def main():
         },
         "mrays_per_s": round(rays_per_step / step_s / 1e6, 3),
         "rays_per_path": round(rays_per_step / paths_per_step, 4),
+        "extend_simd": extend_simd(cnt),
         "whole_path_tflops": round(total_flop / step_s / 1e12, 4),
         "whole_path_frac_fp32_peak": round(total_flop / step_s / 1e12 / PEAK_FP32_TFLOPS, 5),
         "roofline": {

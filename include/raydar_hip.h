@@ -250,6 +250,14 @@ typedef struct rd_counters {
   uint64_t misses;         /* sky samples (world.rs:17-34)                 */
   uint64_t accumulates;    /* per-path accumulations (cpu.rs:204-209)      */
   uint64_t tonemapped_px;  /* pixels written by the tonemap                */
+  /* SIMD efficiency of the closest-hit kernel (wave = 64 lanes):           */
+  uint64_t wave_steps;     /* traversal iterations of waves with a ray     */
+  uint64_t lane_steps;     /* lanes holding a ray, summed over those       */
+  uint64_t sort_steps;     /* wave iterations that ordered two or more
+                              entered inner children                       */
+  uint64_t prim_trips;     /* wave iterations of the primitive loop        */
+  uint64_t cube_trips;     /* those in which a lane tested a cube          */
+  uint64_t refills;        /* times a wave handed new rays to idle lanes   */
 } rd_counters;
 rd_status rd_get_counters(const rd_renderer *r, rd_counters *out);
 rd_status rd_reset_counters(rd_renderer *r);

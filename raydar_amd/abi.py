@@ -90,7 +90,8 @@ class RdCounters(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in (
         "camera_rays", "segments", "box_tests", "sphere_tests", "cube_tests",
         "sphere_hits", "cube_hits", "glass_scatters", "misses", "accumulates",
-        "tonemapped_px")]
+        "tonemapped_px", "wave_steps", "lane_steps", "sort_steps", "prim_trips", "cube_trips",
+        "refills")]
 
 
 class RdKernelTimes(C.Structure):

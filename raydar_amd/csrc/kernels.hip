@@ -184,7 +184,25 @@ __device__ __forceinline__ DevObject load_object(const DevObject *objects, uint3
 
 struct TraceCounts {
   uint32_t box = 0, sphere = 0, cube = 0;
+  // wave-level events of the lanes kernel, each counted by one lane
+  uint32_t wave_steps = 0, lane_steps = 0, sort_steps = 0, prim_trips = 0, cube_trips = 0;
+  uint32_t refills = 0;
 };
+
+// True in exactly one lane of the lanes currently executing (the lowest).
+__device__ __forceinline__ bool wave_leader() { return lanes_below(__ballot(true)) == 0; }
+
+template <bool COUNT>
+__device__ __forceinline__ void count_trip(TraceCounts &tc, const DevPrim &p) {
+  if (COUNT) {
+    const bool cube = (fbits(p.b[3]) & kTagCube) != 0;
+    const uint64_t cubes = __ballot(cube);
+    if (wave_leader()) {
+      tc.prim_trips++;
+      tc.cube_trips += cubes != 0;
+    }
+  }
+}
 
 template <bool COUNT>
 __device__ __forceinline__ void test_leaf(const RayPre &r, const DevPrim *prims, int32_t code,
@@ -443,6 +461,7 @@ void k_extend_lanes(IterArgs a, int p) {
     const uint64_t idle = __ballot(!active);
     if (more && (__popcll(idle) >= refill || idle == ~0ull)) {
       // hand the wave's next indices to its idle lanes, in lane order
+      if (COUNT && lane_id() == 0) tc.refills++;
       const uint32_t take = min((uint32_t)__popcll(idle), c_end - c_next);
       const uint32_t rank = lanes_below(idle);
       const bool got = !active && rank < take;
@@ -500,6 +519,10 @@ void k_extend_lanes(IterArgs a, int p) {
       if (!more) break;
       continue;
     }
+    if (COUNT) {
+      if (lane_id() == 0) tc.wave_steps++;
+      tc.lane_steps += active;
+    }
     if (active) {
       if constexpr (W4) {
 #ifdef RDH_CHECKED
@@ -555,6 +578,7 @@ void k_extend_lanes(IterArgs a, int p) {
           // the nearest and nothing is pushed
           c0 = i0 ? ch.x : i1 ? ch.y : i2 ? ch.z : ch.w;
         } else {
+          if (COUNT && wave_leader()) tc.sort_steps++;
           float k0 = i0 ? t0 : INFINITY, k1 = i1 ? t1 : INFINITY;
           float k2 = i2 ? t2 : INFINITY, k3 = i3 ? t3 : INFINITY;
           int32_t c1 = ch.y, c2 = ch.z, c3 = ch.w;
@@ -596,6 +620,7 @@ void k_extend_lanes(IterArgs a, int p) {
             } else {
               load_prim(prims, j, &pr);
             }
+            count_trip<COUNT>(tc, pr);
             if (COUNT) {
               if (fbits(pr.b[3]) & kTagCube) tc.cube++;
               else tc.sphere++;
@@ -650,6 +675,7 @@ void k_extend_lanes(IterArgs a, int p) {
             } else {
               load_prim(prims, j, &pr);
             }
+            count_trip<COUNT>(tc, pr);
             if (COUNT) {
               if (fbits(pr.b[3]) & kTagCube) tc.cube++;
               else tc.sphere++;
@@ -690,6 +716,10 @@ void k_extend_lanes(IterArgs a, int p) {
     if (tc.box) atomicAdd(&a.counters->v[C_BOX], (unsigned long long)tc.box);
     if (tc.sphere) atomicAdd(&a.counters->v[C_SPHERE], (unsigned long long)tc.sphere);
     if (tc.cube) atomicAdd(&a.counters->v[C_CUBE], (unsigned long long)tc.cube);
+    const uint32_t w[6] = {tc.wave_steps, tc.lane_steps, tc.sort_steps, tc.prim_trips,
+                           tc.cube_trips, tc.refills};
+    for (int i = 0; i < 6; ++i)
+      if (w[i]) atomicAdd(&a.counters->v[C_WAVE_STEPS + i], (unsigned long long)w[i]);
   }
 }
 

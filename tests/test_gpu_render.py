@@ -369,6 +369,15 @@ def test_event_counts_agree_between_trees():
     assert a["sphere_hits"] + a["cube_hits"] + a["misses"] == a["segments"] == b["segments"]
     for k in ("box_tests", "sphere_tests", "cube_tests"):
         assert a[k] > 0 and b[k] > 0, k
+    # the lanes kernel's SIMD-efficiency counts obey their definitions
+    for c in (a, b):
+        assert 0 < c["wave_steps"] <= c["lane_steps"] <= 64 * c["wave_steps"]
+        assert 0 < c["prim_trips"] and c["cube_trips"] <= c["prim_trips"]
+        assert c["prim_trips"] <= c["sphere_tests"] + c["cube_tests"] <= 64 * c["prim_trips"]
+        assert c["sort_steps"] <= c["wave_steps"] and c["refills"] > 0
+    # a binary step tests two boxes per lane, a 4-wide step up to four
+    assert a["box_tests"] == 2 * a["lane_steps"]
+    assert b["box_tests"] <= 4 * b["lane_steps"] and b["sort_steps"] > 0 == a["sort_steps"]
 
 
 @pytest.mark.parametrize("spp,pipes,queue,lds,used", [
