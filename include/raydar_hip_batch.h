@@ -82,7 +82,16 @@ const char *rd_batch_last_error(const rd_batch *b);
 /* Autotuner: times the sample jobs over block {64,128,256} x queue capacity
  * {2^20, 2^22, 2^23, 2^24} x BVH leaf size {1,2,4,8}, keeps the fastest
  * setting on the batch and returns it; default_ms is the time at the
- * library's defaults (block 256, leaf 1, queue 2^24). */
+ * library's defaults (block 256, leaf 1, queue 2^24).
+ *
+ * Results are kept per device: the key is the GPU (name, architecture, CU
+ * count, memory size, PCI location), the batch's pipe count and solo
+ * threshold, and the shape of the sample mix (per job: object count,
+ * resolution, samples, bounce limit).  Tuning the same mix again on the same
+ * GPU returns the kept result at once (cached = 1) and applies it.  The
+ * cache lives in the process; with RAYDAR_HIP_TUNE_CACHE=<file> set it is
+ * also read from and appended to that file (one line per result), so later
+ * processes on the same machine start tuned. */
 typedef struct rd_tune_result {
   int32_t block;
   int32_t leaf;
@@ -90,6 +99,7 @@ typedef struct rd_tune_result {
   double best_ms;
   double default_ms;
   uint32_t configs_tried;
+  uint32_t cached;        /* 1: taken from the per-device cache, nothing timed */
 } rd_tune_result;
 rd_status rd_batch_autotune(rd_batch *b, const rd_job_desc *jobs, uint32_t n_jobs,
                             rd_tune_result *out);

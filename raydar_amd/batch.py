@@ -22,7 +22,7 @@ class RdBatchStats(C.Structure):
 class RdTuneResult(C.Structure):
     _fields_ = [("block", C.c_int32), ("leaf", C.c_int32), ("queue", C.c_uint32),
                 ("best_ms", C.c_double), ("default_ms", C.c_double),
-                ("configs_tried", C.c_uint32)]
+                ("configs_tried", C.c_uint32), ("cached", C.c_uint32)]
 
 
 class RdServerConfig(C.Structure):

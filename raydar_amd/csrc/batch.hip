@@ -17,6 +17,8 @@
 #include <atomic>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
@@ -956,6 +958,113 @@ rd_status batch_fail(rd_batch *b, rd_status s, const std::string &m) {
   return s;
 }
 
+// ---- autotune results kept per device and mix (rd_batch_autotune) --------
+std::mutex g_tune_mu;
+std::map<std::string, rd_tune_result> g_tune;  // guarded by g_tune_mu
+bool g_tune_file_read = false;                 // guarded by g_tune_mu
+
+const char *tune_file() {
+  const char *f = std::getenv("RAYDAR_HIP_TUNE_CACHE");
+  return f && *f ? f : nullptr;
+}
+
+// Device identity + batch shape + per-job work shape, as one line of text
+// (no tabs or newlines: it is the first field of a cache-file line).
+bool tune_key(const rd_batch *b, const rd_job_desc *jobs, uint32_t n, std::string *key) {
+  int dev = 0;
+  hipDeviceProp_t p;
+  if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&p, dev) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  std::string k;
+  for (const char *c = p.name; *c && k.size() < 64; ++c) k += (*c == '\t' || *c == '\n') ? ' ' : *c;
+  k += "|" + std::string(p.gcnArchName) + "|cu" + std::to_string(p.multiProcessorCount) + "|mem" +
+       std::to_string((unsigned long long)p.totalGlobalMem) + "|pci" + std::to_string(p.pciDomainID) +
+       ":" + std::to_string(p.pciBusID) + ":" + std::to_string(p.pciDeviceID) + "|pipes" +
+       std::to_string(b->st.m.size()) + "|solo" + std::to_string((unsigned long long)b->solo.min_paths) +
+       "|graph" + std::to_string(b->st.first().use_graph) + "|jobs" + std::to_string(n);
+  // FNV-1a over each job's work shape
+  uint64_t h = 1469598103934665603ull;
+  auto mix = [&](uint64_t v) {
+    for (int i = 0; i < 8; ++i) {
+      h ^= (v >> (8 * i)) & 0xffu;
+      h *= 1099511628211ull;
+    }
+  };
+  for (uint32_t i = 0; i < n; ++i) {
+    const rd_scene &sc = *jobs[i].scene;
+    mix(sc.object_count);
+    mix(((uint64_t)sc.camera.resolution_x << 32) | sc.camera.resolution_y);
+    mix(((uint64_t)jobs[i].samples << 32) | jobs[i].max_bounces);
+    mix(jobs[i].pass_samples);
+  }
+  char hex[17];
+  std::snprintf(hex, sizeof hex, "%016llx", (unsigned long long)h);
+  *key = k + "|mix" + hex;
+  return true;
+}
+
+// Reads RAYDAR_HIP_TUNE_CACHE once per process: lines of
+// key TAB block TAB leaf TAB queue TAB best_ms TAB default_ms TAB configs;
+// malformed lines are skipped.  Caller holds g_tune_mu.
+void tune_read_file() {
+  if (g_tune_file_read) return;
+  g_tune_file_read = true;
+  const char *f = tune_file();
+  if (!f) return;
+  FILE *fp = std::fopen(f, "r");
+  if (!fp) return;
+  char line[1024];
+  while (std::fgets(line, sizeof line, fp)) {
+    char *tab = std::strchr(line, '\t');
+    if (!tab) continue;
+    *tab = 0;
+    rd_tune_result r{};
+    unsigned q = 0, cfg = 0;
+    if (std::sscanf(tab + 1, "%d\t%d\t%u\t%lf\t%lf\t%u", &r.block, &r.leaf, &q, &r.best_ms,
+                    &r.default_ms, &cfg) != 6)
+      continue;
+    if ((r.block != 64 && r.block != 128 && r.block != 256) || r.leaf < 1 || r.leaf > 8 ||
+        q < 1024 || q > (1u << 28))
+      continue;
+    r.queue = q;
+    r.configs_tried = cfg;
+    g_tune[line] = r;
+  }
+  std::fclose(fp);
+}
+
+bool tune_lookup(const std::string &key, rd_tune_result *r) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  tune_read_file();
+  auto it = g_tune.find(key);
+  if (it == g_tune.end()) return false;
+  *r = it->second;
+  return true;
+}
+
+void tune_store(const std::string &key, const rd_tune_result &r) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  tune_read_file();
+  g_tune[key] = r;
+  if (const char *f = tune_file()) {
+    if (FILE *fp = std::fopen(f, "a")) {
+      std::fprintf(fp, "%s\t%d\t%d\t%u\t%.6f\t%.6f\t%u\n", key.c_str(), r.block, r.leaf, r.queue,
+                   r.best_ms, r.default_ms, r.configs_tried);
+      std::fclose(fp);
+    }
+  }
+}
+
+void apply_tuning(rd_batch *b, const rd_tune_result &r) {
+  b->st.each([&](rdh::Streamer &s) {
+    s.eng.opt.block = r.block;
+    s.eng.opt.leaf = r.leaf;
+    if (!s.eng.q_alloc) s.eng.opt.queue = r.queue;
+  });
+}
+
 bool engine_option(rdh::Streamer &st, const std::string &k, int64_t v, std::string *err) {
   rdh::EngineOptions &o = st.eng.opt;
   if (k == "block") {
@@ -1244,6 +1353,17 @@ const char *rd_batch_last_error(const rd_batch *b) { return b ? b->last_error.c_
 
 rd_status rd_batch_autotune(rd_batch *b, const rd_job_desc *jobs, uint32_t n, rd_tune_result *out) {
   if (!b || !jobs || !n || !out) return RD_ERR_INVALID_ARGUMENT;
+  for (uint32_t i = 0; i < n; ++i)
+    if (!jobs[i].scene) return batch_fail(b, RD_ERR_INVALID_ARGUMENT, "job without a scene");
+  std::string key;
+  if (!tune_key(b, jobs, n, &key)) return batch_fail(b, RD_ERR_DEVICE, "hipGetDeviceProperties failed");
+  rd_tune_result kept{};
+  if (tune_lookup(key, &kept)) {
+    apply_tuning(b, kept);
+    kept.cached = 1;
+    *out = kept;
+    return RD_OK;
+  }
   // The queue capacity and radiance arena are fixed once allocated, so the
   // sweep runs on fresh batches and the winner is applied to `b`.
   const int blocks[] = {64, 128, 256};
@@ -1291,11 +1411,8 @@ rd_status rd_batch_autotune(rd_batch *b, const rd_job_desc *jobs, uint32_t n, rd
   }
   r.best_ms = best;
   r.default_ms = dflt;
-  b->st.each([&](rdh::Streamer &s) {
-    s.eng.opt.block = r.block;
-    s.eng.opt.leaf = r.leaf;
-    if (!s.eng.q_alloc) s.eng.opt.queue = r.queue;
-  });
+  apply_tuning(b, r);
+  tune_store(key, r);
   *out = r;
   return RD_OK;
 }

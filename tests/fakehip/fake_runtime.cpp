@@ -281,11 +281,17 @@ hipError_t hipGetDeviceCount(int *n) {
   return hipSuccess;
 }
 hipError_t hipSetDevice(int d) { return ret(d == 0 ? hipSuccess : hipErrorInvalidValue); }
+hipError_t hipGetDevice(int *d) {
+  *d = 0;
+  return hipSuccess;
+}
 hipError_t hipGetDeviceProperties(hipDeviceProp_t *p, int) {
   std::memset(p, 0, sizeof(*p));
   std::snprintf(p->name, sizeof(p->name), "fake MI355X");
+  std::snprintf(p->gcnArchName, sizeof(p->gcnArchName), "gfx950:sramecc+:xnack-");
   p->multiProcessorCount = 256;
   p->totalGlobalMem = 288ull << 30;
+  p->pciBusID = 5;
   return hipSuccess;
 }
 hipError_t hipDeviceGetAttribute(int *v, hipDeviceAttribute_t, int) {

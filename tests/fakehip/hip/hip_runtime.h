@@ -65,8 +65,10 @@ typedef void *hipGraphNode_t;
 
 struct hipDeviceProp_t {
   char name[256];
+  char gcnArchName[256];
   int multiProcessorCount;
   size_t totalGlobalMem;
+  int pciDomainID, pciBusID, pciDeviceID;
 };
 
 struct float4 {
@@ -85,6 +87,7 @@ inline uchar4 make_uchar4(unsigned char x, unsigned char y, unsigned char z, uns
 
 hipError_t hipGetDeviceCount(int *n);
 hipError_t hipSetDevice(int d);
+hipError_t hipGetDevice(int *d);
 hipError_t hipGetDeviceProperties(hipDeviceProp_t *p, int d);
 hipError_t hipDeviceGetAttribute(int *v, hipDeviceAttribute_t a, int d);
 hipError_t hipDeviceSynchronize();

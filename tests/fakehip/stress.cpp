@@ -328,6 +328,21 @@ void autotune() {
   rd_tune_result t{};
   if (rd_batch_autotune(b, d.data(), 8, &t) != RD_OK) fail(std::string("autotune: ") + rd_batch_last_error(b));
   if (t.configs_tried != 48) fail("autotune tried " + std::to_string(t.configs_tried));
+  std::printf("autotune cached=%u block=%d leaf=%d queue=%u\n", t.cached, t.block, t.leaf, t.queue);
+  // (with RAYDAR_HIP_TUNE_CACHE naming the file of an earlier run, both mixes
+  // come from the file: the first call's cached flag above says which)
+  // the same mix again on the same device: the kept result, nothing timed
+  const uint64_t kernels = fakehip::stats().kernels;
+  rd_tune_result t2{};
+  if (rd_batch_autotune(b, d.data(), 8, &t2) != RD_OK) fail("second autotune");
+  if (!t2.cached || t2.block != t.block || t2.leaf != t.leaf || t2.queue != t.queue)
+    fail("second autotune of the same mix was not served from the cache");
+  if (fakehip::stats().kernels != kernels) fail("cached autotune launched kernels");
+  // another mix is tuned afresh (unless an earlier run's file holds it)
+  rd_tune_result t3{};
+  if (rd_batch_autotune(b, d.data(), 7, &t3) != RD_OK || t3.cached != t.cached)
+    fail("a different mix: cached " + std::to_string(t3.cached));
+  std::printf("autotune other mix cached=%u\n", t3.cached);
   run_batch(b, jobs, "batch after autotune", false);
   rd_batch_destroy(b);
 }

@@ -70,3 +70,17 @@ def test_host_code_on_fake_device_asan(asan, args, nosync):
 @pytest.mark.parametrize("args", [["plain"], ["server", "--seconds", "6", "--speed", "10"]])
 def test_host_code_on_fake_device_tsan(tsan, args):
     _run(tsan, args, 900)
+
+
+def test_autotune_results_kept_per_device_across_processes(asan, tmp_path):
+    # the first process tunes both mixes and appends them to the file; the
+    # second finds both there and times nothing (stress.cpp checks that a
+    # repeated call within a process launches no kernel)
+    f = str(tmp_path / "tune.txt")
+    first = _run(asan, ["autotune"], 900, RAYDAR_HIP_TUNE_CACHE=f)
+    assert "autotune cached=0" in first and "other mix cached=0" in first
+    lines = open(f).read().splitlines()
+    assert len(lines) == 2 and all("gfx950" in l and l.count("\t") == 6 for l in lines)
+    second = _run(asan, ["autotune"], 900, RAYDAR_HIP_TUNE_CACHE=f)
+    assert "autotune cached=1" in second and "other mix cached=1" in second
+    assert len(open(f).read().splitlines()) == 2

@@ -182,7 +182,10 @@ def test_autotune_returns_a_valid_setting():
     with BatchRenderer() as br:
         r = br.autotune(jobs)
         assert r["block"] in (64, 128, 256) and r["leaf"] in (1, 2, 4, 8)
-        assert r["configs_tried"] == 48 and r["best_ms"] <= r["default_ms"]
+        assert r["configs_tried"] == 48 and r["best_ms"] <= r["default_ms"] and not r["cached"]
+        again = br.autotune(jobs)  # the same mix on the same GPU: kept, nothing timed
+        assert again["cached"] and {k: again[k] for k in ("block", "leaf", "queue")} == \
+            {k: r[k] for k in ("block", "leaf", "queue")}
         out = [Job(scenegen.plain_small(0), 2, 4, 0)]
         br.render(out)
     np.testing.assert_array_equal(out[0].acc, _oracle_acc(scenegen.plain_small(0), 2, 4, 0))
