@@ -393,16 +393,19 @@ RD_HD ScatterOut scatter(F3 ray_d, const HitRec &h, const DevObject &ob, Draws &
   } else if (unit_f32(dr.slot(2)) < rough) {
     pick = kDiffuse;
   }
+  // Every branch offsets by the same unit-sphere draw, and the refracted and
+  // the reflected direction are normalized alike: computing the draw once
+  // and sharing the normalization keeps a wave whose lanes take different
+  // branches from running those parts once per branch (same operations per
+  // lane, so the same bits).
+  const F3 u = unit_sphere_draw(dr, 4);
   F3 dir;
-  if (pick == kRefract) {
-    const F3 refr = refract(rd, h.n, eta);
-    dir = normalize(add(refr, scale(unit_sphere_draw(dr, 4), rough)));
-  } else if (pick == kDiffuse) {
-    dir = add(h.n, unit_sphere_draw(dr, 4));
+  if (pick == kDiffuse) {
+    dir = add(h.n, u);
     if (dot(dir, h.n) < 0.0f) dir = neg(dir);
   } else {
-    const F3 perfect = reflect(ray_d, h.n);
-    dir = normalize(add(perfect, scale(unit_sphere_draw(dr, 4), rough)));
+    const F3 base = pick == kRefract ? refract(rd, h.n, eta) : reflect(ray_d, h.n);
+    dir = normalize(add(base, scale(u, rough)));
   }
   ScatterOut out;
   const F3 off = trans ? dir : h.n;
