@@ -1090,7 +1090,7 @@ bool engine_option(rdh::Streamer &st, const std::string &k, int64_t v, std::stri
     if (v < 2 || v > 1024) { *err = "iters_per_pump out of range"; return false; }
     st.iters_per_pump = (int)v;
   } else if (k == "l_capacity") {
-    if (v < (1 << 16) || v > (1ll << 32)) { *err = "l_capacity out of range"; return false; }
+    if (v < (1 << 16) || v > (1ll << 31)) { *err = "l_capacity out of range"; return false; }
     if (st.eng.l_alloc && (uint64_t)v != st.eng.l_alloc) { *err = "l_capacity is fixed once rendering started"; return false; }
     o.l_capacity = (uint64_t)v;
   } else if (k == "traversal") {

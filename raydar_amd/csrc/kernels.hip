@@ -768,12 +768,15 @@ void k_shade(IterArgs a, int p) {
       const uint32_t tb = q.tb[k];
       ti = chk(ticket_of<SINGLE>(tb), a.ticket_cap, K_TICKET, a.ctl);
       bounce = tb & 0xffu;
+      const uint32_t pw = q.pidx[k];
+      pidx = pw & kPathIndexMask;
       F3 at = f3(1.0f, 1.0f, 1.0f), li = f3(0.0f, 0.0f, 0.0f);
       if (bounce != 0) {
         at = f3(q.ax[k], q.ay[k], q.az[k]);
-        li = f3(q.lx[k], q.ly[k], q.lz[k]);
+        // light stays exactly (+0, +0, +0) until an emitter or the sky adds
+        // to it; only lit paths carry it in the queue
+        if (pw & kLitFlag) li = f3(q.lx[k], q.ly[k], q.lz[k]);
       }
-      pidx = q.pidx[k];
       const DevTicket &T = a.tickets[ti];
       const DevJob &J = a.jobs[SINGLE ? 0u : chk(T.job, a.job_cap, K_JOB, a.ctl)];
       l_slot = chk((unsigned long long)(T.l_offset + pidx), a.l_cap, K_RADIANCE, a.ctl);
@@ -829,9 +832,12 @@ void k_shade(IterArgs a, int p) {
       nq.ox[k] = no.x; nq.oy[k] = no.y; nq.oz[k] = no.z;
       nq.dx[k] = nd.x; nq.dy[k] = nd.y; nq.dz[k] = nd.z;
       nq.ax[k] = na.x; nq.ay[k] = na.y; nq.az[k] = na.z;
-      nq.lx[k] = nl.x; nq.ly[k] = nl.y; nq.lz[k] = nl.z;
+      const bool lit = (fbits(nl.x) | fbits(nl.y) | fbits(nl.z)) != 0u;
+      if (lit) {
+        nq.lx[k] = nl.x; nq.ly[k] = nl.y; nq.lz[k] = nl.z;
+      }
       nq.tb[k] = (ti << 8) | bounce;
-      nq.pidx[k] = pidx;
+      nq.pidx[k] = pidx | (lit ? kLitFlag : 0u);
     }
     if (finished) a.L[l_slot] = make_float4(nl.x, nl.y, nl.z, 1.0f);
     if (a.track_done) {

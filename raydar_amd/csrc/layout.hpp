@@ -29,6 +29,12 @@ constexpr int kMaxLeafSize = 8;
 constexpr int kStackDepth = 32;          // >= max BVH depth the builder emits
 constexpr uint32_t kNarrowStackNodes = 65535;  // 16-bit LDS stack entries suffice
 constexpr uint32_t kNoHit = 0x7fffffffu; // best-id sentinel: "no hit yet"
+// Path queue word `pidx`: the path's index in its ticket in bits 0..30 (a
+// pass holds at most 2^31 paths: l_capacity <= 2^31), and in bit 31 whether
+// the path's light is not (+0, +0, +0); the light words of a queue slot are
+// written and read only when it is set.
+constexpr uint32_t kLitFlag = 0x80000000u;
+constexpr uint32_t kPathIndexMask = 0x7fffffffu;
 constexpr int kDrawWords = 8;            // random words per bounce (two Philox blocks)
 
 // Leaf child code: bit 31 set, first prim in bits 4..30, count in bits 0..3.
