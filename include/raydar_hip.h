@@ -224,6 +224,8 @@ rd_status rd_read_accumulator(rd_renderer *r, float *out, size_t float_capacity)
  *                 idle lanes once k or more are idle (default 48)
  *   "lds_bvh"     scenes whose BVH nodes + primitives take at most this many
  *                 bytes are traversed from LDS (default 14336; 0 = never)
+ *   "lds_objects" scenes whose objects (64 B each) take at most this many
+ *                 bytes are shaded from a copy in LDS (default 16384; 0 = never)
  *   "bvh4"        1 = traverse the tree collapsed to 4-wide nodes (default),
  *                 0 = the binary tree
  *   "pipes"       a pass of k >= 2 samples is divided by samples over up to

@@ -287,6 +287,7 @@ bool option_value(const rd_renderer *r, const std::string &k, int64_t *v) {
   else if (k == "traversal") *v = o.brute_force;
   else if (k == "refill") *v = o.refill;
   else if (k == "lds_bvh") *v = o.lds_bvh;
+  else if (k == "lds_objects") *v = o.lds_objects;
   else if (k == "shade_blocks") *v = o.shade_blocks;
   else if (k == "bvh4") *v = o.bvh4;
   else if (k == "pipes") *v = o.pipes;
@@ -652,6 +653,9 @@ rd_status rd_set_option(rd_renderer *r, const char *key, int64_t v) {
   } else if (k == "lds_bvh") {
     if (v < 0 || v > (1 << 17)) return r->fail(RD_ERR_INVALID_ARGUMENT, "lds_bvh must be in 0..131072");
     o.lds_bvh = (uint32_t)v;
+  } else if (k == "lds_objects") {
+    if (v < 0 || v > (1 << 16)) return r->fail(RD_ERR_INVALID_ARGUMENT, "lds_objects must be in 0..65536");
+    o.lds_objects = (uint32_t)v;
   } else if (k == "count") {
     o.count = v != 0;
   } else if (k == "kernel_timing") {
@@ -910,6 +914,7 @@ rd_status rd_debug_extend(const rd_scene *scene, int leaf, const rd_extend_varia
   a.wide_stack = v->wide ? 1 : 0;
   a.refill = v->refill;
   a.lds_bvh_bytes = lds_bytes;
+  a.lds_obj_bytes = 0;
   a.stack_rows = rows;
   a.bvh4 = v->bvh4 ? 1 : 0;
   a.slots = (uint32_t)slots;

@@ -331,6 +331,7 @@ IterArgs Engine::iter_args(const DevJob *d_jobs) const {
   a.refill = opt.refill;
   a.lds_bvh_bytes = 0;
   a.stack_rows = 0;
+  a.lds_obj_bytes = 0;
   a.bvh4 = 0;
   a.slots = (uint32_t)kSegments * seg_cap;
   a.ticket_cap = ticket_alloc;
@@ -352,7 +353,8 @@ LaunchShape Engine::shape(int which, const IterArgs *a) const {
     const uint32_t dyn = (which == 1 && a) ? extend_dynamic_lds(*a, s.block) : 0;
     const bool w4 = which == 1 && a && a->bvh4;
     const bool single = a ? a->single_job != 0 : true;
-    int per_cu = std::max(1, occupancy_blocks(which, s.block, lanes, dyn ? a->lds_bvh_bytes : 0, dyn, w4, single));
+    const uint32_t dyn_lds = which == 2 && a ? shade_dynamic_lds(*a) : dyn;
+    int per_cu = std::max(1, occupancy_blocks(which, s.block, lanes, dyn ? a->lds_bvh_bytes : 0, dyn_lds, w4, single));
     if (which == 2 && opt.shade_blocks > 0) per_cu = std::min(per_cu, opt.shade_blocks);
     s.grid = num_cus * per_cu;
   }
@@ -570,6 +572,12 @@ bool Engine::run_tickets(const DevJob *d_jobs, std::vector<DevTicket> tickets,
           a.lds_bvh_bytes = (uint32_t)bytes;
           a.stack_rows = rows;
         }
+      }
+      if (a.single_job) {
+        // a small scene's objects are read by shade from LDS: the object load
+        // depends on the hit id, so from L2 it is a second dependent fetch
+        const uint64_t ob = (uint64_t)sizeof(DevObject) * host_jobs[0].n_prims;
+        if (ob > 0 && ob <= opt.lds_objects) a.lds_obj_bytes = (uint32_t)ob;
       }
       const uint64_t cap_iters = (wk / e->q_alloc + 2) * (uint64_t)(max_bounces + 1) + 64;
       pipes.push_back(Pipe{e, a, wk, 0, 0, cap_iters, false, kNoBound});

@@ -73,6 +73,8 @@ struct EngineOptions {
   uint32_t lds_bvh = 14336;       // single scenes with at most this many BVH bytes
                                   // are traversed from LDS (0: never)
   int shade_blocks = 0;           // shade workgroups per CU (0: as many as fit)
+  uint32_t lds_objects = 16384;   // single scenes with at most this many bytes of
+                                  // objects (64 B each) are shaded from LDS (0: never)
   int bvh4 = 1;                   // single scenes: traverse the 4-wide tree
   int leaf = 1;                   // BVH leaf size; 1 measured fastest for one scene
   int count = 0;

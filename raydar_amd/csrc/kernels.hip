@@ -171,15 +171,17 @@ __device__ __forceinline__ void load_prim(const DevPrim *prims, uint32_t j, DevP
   p->b[0] = b.x; p->b[1] = b.y; p->b[2] = b.z; p->b[3] = b.w;
 }
 
-__device__ __forceinline__ DevObject load_object(const DevObject *objects, uint32_t id) {
-  GlobalF4 *s = (GlobalF4 *)(objects + id);
+__device__ __forceinline__ DevObject object_from(VecF4 g, VecF4 m0, VecF4 m1, VecF4 m2) {
   DevObject o;
-  const VecF4 g = s[0], m0 = s[1], m1 = s[2], m2 = s[3];
   o.geo[0] = g.x; o.geo[1] = g.y; o.geo[2] = g.z; o.geo[3] = g.w;
   o.m0[0] = m0.x; o.m0[1] = m0.y; o.m0[2] = m0.z; o.m0[3] = m0.w;
   o.m1[0] = m1.x; o.m1[1] = m1.y; o.m1[2] = m1.z; o.m1[3] = m1.w;
   o.m2[0] = m2.x; o.m2[1] = m2.y; o.m2[2] = m2.z; o.m2[3] = m2.w;
   return o;
+}
+__device__ __forceinline__ DevObject load_object(const DevObject *objects, uint32_t id) {
+  GlobalF4 *s = (GlobalF4 *)(objects + id);
+  return object_from(s[0], s[1], s[2], s[3]);
 }
 
 struct TraceCounts {
@@ -748,7 +750,17 @@ void k_shade(IterArgs a, int p) {
     done.ticket[threadIdx.x] = 0xffffffffu;
     done.count[threadIdx.x] = 0;
   }
-  make_plan(a, p, pl);  // its barriers also publish the table's reset
+  // A small scene's objects are copied into LDS (lds_obj_bytes, launch
+  // constant): the object read follows the hit id, and from LDS it does not
+  // add a second global-memory latency to every path.
+  extern __shared__ __align__(16) unsigned char dynamic_lds[];
+  const bool lds_obj = SINGLE && a.lds_obj_bytes != 0;
+  VecF4 *lds_objects = reinterpret_cast<VecF4 *>(dynamic_lds);
+  if (lds_obj) {
+    GlobalF4 *g = (GlobalF4 *)a.jobs[0].objects;
+    for (uint32_t v = threadIdx.x; v < a.lds_obj_bytes / 16u; v += BLOCK) lds_objects[v] = g[v];
+  }
+  make_plan(a, p, pl);  // its barriers also publish the table's reset and the LDS copy
   const PathQueue q = queue_of(a, p), nq = queue_of(a, 1 - p);
   const uint32_t lane = lane_id();
   const uint32_t wid = threadIdx.x >> 6;
@@ -782,7 +794,10 @@ void k_shade(IterArgs a, int p) {
       l_slot = chk((unsigned long long)(T.l_offset + pidx), a.l_cap, K_RADIANCE, a.ctl);
       const uint32_t id = a.hit_id[k];
       if (id != kNoHit) {
-        const DevObject ob = load_object(J.objects, chk(id, J.n_prims, K_OBJECT, a.ctl));
+        const uint32_t oid = chk(id, J.n_prims, K_OBJECT, a.ctl);
+        const DevObject ob = lds_obj ? object_from(lds_objects[4 * oid], lds_objects[4 * oid + 1],
+                                                   lds_objects[4 * oid + 2], lds_objects[4 * oid + 3])
+                                     : load_object(J.objects, oid);
         const HitRec h = closest_hit(o, d, a.hit_t[k], ob);
         const uint32_t npix = J.width * J.height;
         BounceDraws dr;
@@ -1105,9 +1120,11 @@ uint32_t extend_dynamic_lds(const IterArgs &a, int block) {
   return ((a.stack_rows * (uint32_t)block * 2u + 15u) & ~15u) + a.lds_bvh_bytes;
 }
 
+uint32_t shade_dynamic_lds(const IterArgs &a) { return a.single_job ? a.lds_obj_bytes : 0u; }
+
 static hipError_t launch_stage(int which, const IterArgs &a, int parity, const LaunchShape &s,
                                bool count, hipStream_t st) {
-  const uint32_t dyn = which == 1 ? extend_dynamic_lds(a, s.block) : 0;
+  const uint32_t dyn = which == 1 ? extend_dynamic_lds(a, s.block) : which == 2 ? shade_dynamic_lds(a) : 0;
   const void *fn = kernel_fn(which, s.block, count, a.single_job != 0, a.wide_stack != 0,
                              a.refill > 0, dyn > 0, a.bvh4 != 0);
   if (!fn) return hipErrorInvalidValue;

@@ -43,6 +43,8 @@ struct IterArgs {
   uint32_t lds_bvh_bytes;           // > 0: single job whose nodes + primitives (this many
                                     // bytes) each extend workgroup copies into LDS
   uint32_t stack_rows;              // traversal stack entries per lane with lds_bvh_bytes
+  uint32_t lds_obj_bytes;           // > 0: single job whose objects (this many bytes)
+                                    // each shade workgroup copies into LDS
   int bvh4;                         // single job: traverse DevJob::nodes4
   // Buffer sizes, read only by bounds-checked builds (RDH_CHECKED):
   uint32_t slots;                   // entries of each queue array and hit array
@@ -53,6 +55,8 @@ struct IterArgs {
 
 // Dynamic LDS of the extend launch for these arguments (0: static only).
 uint32_t extend_dynamic_lds(const IterArgs &a, int block);
+// Dynamic LDS of the shade launch (the job's objects, or 0).
+uint32_t shade_dynamic_lds(const IterArgs &a);
 
 struct LaunchShape {
   int block = 256;   // 64, 128 or 256
