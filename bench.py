@@ -164,21 +164,37 @@ def batched_vs_plain(reps=2, gap=0.0, cpu=False):
                           "options": {k: br.get_option(k) for k in ("block", "queue", "leaf", "pipes",
                                                                     "solo_paths")},
                           "solo_jobs": br.get_option("solo_jobs")}
+    solo_paths = out["batched"]["options"]["solo_paths"]
     with ra.HipRenderer() as r:
         runs = []
+        large_s = [0.0] * (reps + 1)  # time of the jobs the batch gives the solo lane
         for k in range(reps + 1):  # the first pass warms up, as for the batch
             time.sleep(gap)
             t0 = time.perf_counter()
             for j in specs:
+                tj = time.perf_counter()
                 r.max_sample_count = j.spp
                 r.max_bounces = j.bounces
                 r.set_seed(j.seed)
                 r.render_frame(j.scene)
+                if solo_paths and j.paths >= solo_paths:
+                    large_s[k] += time.perf_counter() - tj
             if k:
                 runs.append(time.perf_counter() - t0)
         dt_p = sum(runs) / len(runs)
+        dt_large = sum(large_s[1:]) / reps
+    n_large = sum(1 for j in specs if solo_paths and j.paths >= solo_paths)
     out["sequential_plain"] = {"seconds": round(dt_p, 4), "mpaths_per_s": round(paths / dt_p / 1e6, 2),
-                               "runs_s": [round(t, 4) for t in runs]}
+                               "runs_s": [round(t, 4) for t in runs],
+                               # the batch can beat the sequential renders by at most this
+                               # much if its small jobs cost nothing and its large ones run
+                               # as fast as plain renders
+                               "large_jobs": n_large,
+                               "large_job_paths": sum(j.paths for j in specs
+                                                      if solo_paths and j.paths >= solo_paths),
+                               "large_jobs_s": round(dt_large, 4),
+                               "bound_if_small_jobs_were_free": round(dt_p / dt_large, 3)
+                               if dt_large > 0 else None}
     out["batched_over_plain"] = round(dt_p / dt_b, 3)
     if cpu:
         print("# cpu baseline on the batch...", file=sys.stderr, flush=True)

@@ -295,6 +295,12 @@ void fake_shade(const IterArgs a, int p, const LaunchShape sh) {
   fakehip::count_kernel();
   std::lock_guard<std::mutex> g(g_k);
   if (!check_args(a, sh, "shade")) return;
+  // LDS-resident objects: only for one job, and exactly that job's objects
+  // (the kernel copies lds_obj_bytes from its object array)
+  if (a.lds_obj_bytes &&
+      (!a.single_job || a.lds_obj_bytes != sizeof(DevObject) * (uint64_t)a.jobs[0].n_prims ||
+       a.lds_obj_bytes > 65536))
+    err("shade", "LDS object copy of " + std::to_string(a.lds_obj_bytes) + " bytes does not match the job");
   const Plan pl = plan(a, p);
   std::vector<Run> &runs = g_queue[queue_of(a, p).tb];
   std::vector<Run> &next = g_queue[queue_of(a, 1 - p).tb];
@@ -384,6 +390,8 @@ uint32_t extend_dynamic_lds(const IterArgs &a, int block) {
   if (a.refill <= 0 || a.lds_bvh_bytes == 0 || !a.single_job || a.wide_stack) return 0;
   return ((a.stack_rows * (uint32_t)block * 2u + 15u) & ~15u) + a.lds_bvh_bytes;
 }
+
+uint32_t shade_dynamic_lds(const IterArgs &a) { return a.single_job ? a.lds_obj_bytes : 0u; }
 
 int occupancy_blocks(int which, int block, bool, uint32_t, uint32_t, bool, bool single) {
   const int waves = which == 1 ? 8 : which == 2 ? (single ? 7 : 5) : 8;
