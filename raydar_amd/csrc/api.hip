@@ -504,6 +504,7 @@ rd_status rd_render_frame_noalloc(rd_renderer *r, const rd_scene *scene, uint8_t
   ds.n_nodes = (uint32_t)b.nodes.size();
   ds.n_nodes4 = (uint32_t)b.nodes4.size();
   ds.stack4 = b.stack4;
+  ds.max_leaf = b.max_leaf;
   ds.n_prims = (uint32_t)b.prims.size();
   ds.r0 = b.r0;
   ds.depth = b.depth;
@@ -917,6 +918,7 @@ rd_status rd_debug_extend(const rd_scene *scene, int leaf, const rd_extend_varia
   a.lds_obj_bytes = 0;
   a.stack_rows = rows;
   a.bvh4 = v->bvh4 ? 1 : 0;
+  a.leaf1 = single && v->bvh4 && b.max_leaf == 1 ? 1 : 0;
   a.slots = (uint32_t)slots;
   a.ticket_cap = (uint32_t)v->jobs;
   a.job_cap = (uint32_t)v->jobs;

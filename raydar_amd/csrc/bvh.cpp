@@ -86,6 +86,7 @@ struct Builder {
   std::vector<uint32_t> order;
   int leaf_size = 4;
   uint32_t max_depth_seen = 0;
+  uint32_t max_leaf = 0;
   double sah = 0.0;
   static constexpr int kBins = 16;
   static constexpr int kMaxDepth = kStackDepth - 2;
@@ -124,6 +125,7 @@ struct Builder {
 
   int32_t make_leaf(size_t b, size_t e, uint32_t depth) {
     const uint32_t first = (uint32_t)order.size();
+    max_leaf = std::max(max_leaf, (uint32_t)(e - b));
     for (size_t i = b; i < e; ++i) order.push_back(items[i].id);
     max_depth_seen = std::max(max_depth_seen, depth);
     return leaf_code(first, (uint32_t)(e - b));
@@ -465,6 +467,7 @@ bool build_scene(const rd_scene &scene, int leaf_size, BuiltScene *out, std::str
   }
   out->nodes = std::move(bld.nodes);
   out->depth = bld.max_depth_seen;
+  out->max_leaf = bld.max_leaf;
   out->nodes4.clear();
   out->depth4 = 0;
   out->stack4 = 0;

@@ -21,6 +21,7 @@ struct BuiltScene {
   std::vector<DevNode4> nodes4;    // the same tree, collapsed to 4-wide nodes
   uint32_t depth4 = 0;             // levels of nodes4
   uint32_t stack4 = 0;             // traversal stack entries nodes4 can need
+  uint32_t max_leaf = 0;           // primitives in the largest leaf
   std::vector<DevPrim> prims;      // BVH leaf order
   std::vector<DevObject> objects;  // original order
   uint32_t depth = 0;              // max node depth (root = 1)

@@ -279,6 +279,7 @@ bool Engine::upload_scene(const BuiltScene &b, DeviceScene *out, std::string *er
   out->n_nodes = (uint32_t)b.nodes.size();
   out->n_nodes4 = (uint32_t)b.nodes4.size();
   out->stack4 = b.stack4;
+  out->max_leaf = b.max_leaf;
   out->n_prims = (uint32_t)b.prims.size();
   out->r0 = b.r0;
   out->depth = b.depth;
@@ -333,6 +334,7 @@ IterArgs Engine::iter_args(const DevJob *d_jobs) const {
   a.stack_rows = 0;
   a.lds_obj_bytes = 0;
   a.bvh4 = 0;
+  a.leaf1 = 0;
   a.slots = (uint32_t)kSegments * seg_cap;
   a.ticket_cap = ticket_alloc;
   a.job_cap = 1;
@@ -565,6 +567,7 @@ bool Engine::run_tickets(const DevJob *d_jobs, std::vector<DevTicket> tickets,
         // the traversal can need.
         const DevJob &j = host_jobs[0];
         a.bvh4 = opt.bvh4 && j.n_nodes4 > 0 && j.stack4 + 3 <= (uint32_t)kStackDepth ? 1 : 0;
+        a.leaf1 = a.bvh4 && j.max_leaf == 1 ? 1 : 0;
         const uint64_t nb = a.bvh4 ? (uint64_t)sizeof(DevNode4) * j.n_nodes4 : (uint64_t)sizeof(DevNode) * j.n_nodes;
         const uint64_t bytes = nb + (uint64_t)sizeof(DevPrim) * j.n_prims;
         const uint32_t rows = a.bvh4 ? j.stack4 + 3 : j.depth + 1;
@@ -696,6 +699,7 @@ void fill_job(const rd_scene &s, const DeviceScene &ds, float *acc, uint32_t max
   j->nodes4 = ds.nodes4;
   j->n_nodes4 = ds.n_nodes4;
   j->stack4 = ds.stack4;
+  j->max_leaf = ds.max_leaf;
 }
 
 }  // namespace rdh

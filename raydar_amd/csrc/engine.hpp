@@ -28,6 +28,7 @@ struct DeviceScene {
   DevNode *nodes = nullptr;
   DevNode4 *nodes4 = nullptr;
   uint32_t n_nodes4 = 0, stack4 = 0;
+  uint32_t max_leaf = 0;
   DevPrim *prims = nullptr;
   DevObject *objects = nullptr;
   uint32_t n_nodes = 0, n_prims = 0;

@@ -403,7 +403,10 @@ __device__ __forceinline__ void order_pair(float &ka, int32_t &ca, float &kb, in
 // avoid spilling; every other variant fits 8.
 constexpr int lanes_min_waves(bool single, bool w4) { return !single && w4 ? 7 : RDH_EXTEND_WAVES; }
 
-template <int BLOCK, bool COUNT, bool SINGLE, class StackT, bool LDS, bool W4>
+// LEAF1 (single 4-wide jobs whose leaves hold one primitive each, IterArgs::
+// leaf1): a hit leaf child contributes exactly one primitive, so the leaf
+// counts are the hit flags and each trip's primitive is its child's first.
+template <int BLOCK, bool COUNT, bool SINGLE, class StackT, bool LDS, bool W4, bool LEAF1 = false>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(lanes_min_waves(SINGLE, W4), 8)))
 void k_extend_lanes(IterArgs a, int p) {
   __shared__ StackT static_stack[LDS ? 1 : kStackDepth * BLOCK];
@@ -562,12 +565,15 @@ void k_extend_lanes(IterArgs a, int p) {
         }
         const bool l0 = h0 && is_leaf(ch.x), l1 = h1 && is_leaf(ch.y);
         const bool l2 = h2 && is_leaf(ch.z), l3 = h3 && is_leaf(ch.w);
-        const uint32_t n0 = l0 ? leaf_count(ch.x) : 0u;
-        const uint32_t p1 = n0 + (l1 ? leaf_count(ch.y) : 0u);
-        const uint32_t p2 = p1 + (l2 ? leaf_count(ch.z) : 0u);
-        const uint32_t n_leaf = p2 + (l3 ? leaf_count(ch.w) : 0u);
-        const uint32_t f0 = leaf_first(ch.x), f1 = leaf_first(ch.y) - n0;
-        const uint32_t f2 = leaf_first(ch.z) - p1, f3 = leaf_first(ch.w) - p2;
+        const uint32_t n0 = l0 ? (LEAF1 ? 1u : leaf_count(ch.x)) : 0u;
+        const uint32_t p1 = n0 + (l1 ? (LEAF1 ? 1u : leaf_count(ch.y)) : 0u);
+        const uint32_t p2 = p1 + (l2 ? (LEAF1 ? 1u : leaf_count(ch.z)) : 0u);
+        const uint32_t n_leaf = p2 + (l3 ? (LEAF1 ? 1u : leaf_count(ch.w)) : 0u);
+        // trip t tests f_k + t for the k-th hit leaf (one primitive each
+        // with LEAF1: trip t is that leaf's first primitive)
+        const uint32_t f0 = leaf_first(ch.x), f1 = leaf_first(ch.y) - (LEAF1 ? 0u : n0);
+        const uint32_t f2 = leaf_first(ch.z) - (LEAF1 ? 0u : p1);
+        const uint32_t f3 = leaf_first(ch.w) - (LEAF1 ? 0u : p2);
         // The next node and the stack are settled before the primitive tests
         // (they depend only on this node's boxes), so the children's keys
         // and ids are dead while the primitives are tested.
@@ -611,7 +617,8 @@ void k_extend_lanes(IterArgs a, int p) {
         if (n_leaf) {
           for (uint32_t t = 0; t < n_leaf; ++t) {
             DevPrim pr;
-            uint32_t j = t < n0 ? f0 + t : t < p1 ? f1 + t : t < p2 ? f2 + t : f3 + t;
+            uint32_t j = LEAF1 ? (t < n0 ? f0 : t < p1 ? f1 : t < p2 ? f2 : f3)
+                               : (t < n0 ? f0 + t : t < p1 ? f1 + t : t < p2 ? f2 + t : f3 + t);
 #ifdef RDH_CHECKED
             j = chk(j, cur_prims, K_PRIM, a.ctl);
 #endif
@@ -1059,10 +1066,13 @@ uint32_t blocks_for(uint64_t n, uint32_t block) { return (uint32_t)((n + block -
 // Kernel instance for a launch: block size, counting build, single-job
 // specialisation and stack width are template parameters.
 template <bool COUNT, bool SINGLE>
-static const void *extend_fn(int block, bool wide, bool lanes, bool lds, bool w4) {
+static const void *extend_fn(int block, bool wide, bool lanes, bool lds, bool w4, bool leaf1) {
 #define RDH_EXT(B)                                                                           \
   if (block == B) {                                                                          \
     if (lanes && !wide && w4) {                                                              \
+      if (SINGLE && leaf1)                                                                   \
+        return lds ? (const void *)k_extend_lanes<B, COUNT, true, uint16_t, true, true, true> \
+                   : (const void *)k_extend_lanes<B, COUNT, true, uint16_t, false, true, true>; \
       if (lds && SINGLE) return (const void *)k_extend_lanes<B, COUNT, true, uint16_t, true, true>; \
       return (const void *)k_extend_lanes<B, COUNT, SINGLE, uint16_t, false, true>;          \
     }                                                                                        \
@@ -1088,17 +1098,18 @@ static const void *shade_fn(int block) {
   return nullptr;
 }
 static const void *kernel_fn(int which, int block, bool count, bool single, bool wide,
-                             bool lanes = false, bool lds = false, bool w4 = false) {
+                             bool lanes = false, bool lds = false, bool w4 = false,
+                             bool leaf1 = false) {
   if (which == 0) {
     if (count) return single ? (const void *)k_generate<true, true> : (const void *)k_generate<true, false>;
     return single ? (const void *)k_generate<false, true> : (const void *)k_generate<false, false>;
   }
   if (which == 1) {
     if (count)
-      return single ? extend_fn<true, true>(block, wide, lanes, lds, w4)
-                    : extend_fn<true, false>(block, wide, lanes, lds, w4);
-    return single ? extend_fn<false, true>(block, wide, lanes, lds, w4)
-                  : extend_fn<false, false>(block, wide, lanes, lds, w4);
+      return single ? extend_fn<true, true>(block, wide, lanes, lds, w4, leaf1)
+                    : extend_fn<true, false>(block, wide, lanes, lds, w4, leaf1);
+    return single ? extend_fn<false, true>(block, wide, lanes, lds, w4, leaf1)
+                  : extend_fn<false, false>(block, wide, lanes, lds, w4, leaf1);
   }
   if (count) return single ? shade_fn<true, true>(block) : shade_fn<true, false>(block);
   return single ? shade_fn<false, true>(block) : shade_fn<false, false>(block);
@@ -1126,7 +1137,7 @@ static hipError_t launch_stage(int which, const IterArgs &a, int parity, const L
                                bool count, hipStream_t st) {
   const uint32_t dyn = which == 1 ? extend_dynamic_lds(a, s.block) : which == 2 ? shade_dynamic_lds(a) : 0;
   const void *fn = kernel_fn(which, s.block, count, a.single_job != 0, a.wide_stack != 0,
-                             a.refill > 0, dyn > 0, a.bvh4 != 0);
+                             a.refill > 0, dyn > 0, a.bvh4 != 0, a.leaf1 != 0);
   if (!fn) return hipErrorInvalidValue;
   IterArgs args = a;
   int par = parity;

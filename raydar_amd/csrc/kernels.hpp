@@ -46,6 +46,7 @@ struct IterArgs {
   uint32_t lds_obj_bytes;           // > 0: single job whose objects (this many bytes)
                                     // each shade workgroup copies into LDS
   int bvh4;                         // single job: traverse DevJob::nodes4
+  int leaf1;                        // single 4-wide job whose leaves hold one primitive each
   // Buffer sizes, read only by bounds-checked builds (RDH_CHECKED):
   uint32_t slots;                   // entries of each queue array and hit array
   uint32_t ticket_cap;              // ticket table entries

@@ -104,6 +104,7 @@ struct DevJob {
   uint32_t depth;           // BVH levels (a traversal stack needs fewer entries)
   uint32_t n_nodes4;        // the same tree as 4-wide nodes
   uint32_t stack4;          // stack entries a 4-wide traversal can hold
+  uint32_t max_leaf;        // primitives in the largest leaf (1: one per leaf)
   const DevNode4 *nodes4;
 };
 

@@ -115,6 +115,8 @@ EXTEND_VARIANTS = {
     "w4_refill1_grid3": dict(bvh4=1, refill=1, block=64, grid=3),
     "multi_w4_refill16_grid5": dict(bvh4=1, jobs=5, refill=16, block=128, grid=5),
     "w4_lds_leaf4_grid2": dict(bvh4=1, lds=1, leaf=4, block=64, grid=2),
+    # leaf 1 (the default above) takes the one-primitive-per-leaf kernels; leaf 2 the general ones
+    "w4_global_leaf2": dict(bvh4=1, lds=0, leaf=2),
 }
 
 
