@@ -235,7 +235,19 @@ rd_status rd_read_accumulator(rd_renderer *r, float *out, size_t float_capacity)
  *                 scenes traversed from LDS, else 3); read-only
  *                 "pipes_used" reports the last pass's count
  *   "shade_blocks" shade workgroups per CU (0 = as many as fit)
- *   "count"       1 = collect algorithmic event counters in the next passes */
+ *   "count"       1 = collect algorithmic event counters in the next passes
+ *   "semantics"   0 = the CPU renderer's (default); 1 = the Vulkan renderer's
+ *                 (src/renderer/vulkan.rs, shaders/vulkan/raytrace.rgen):
+ *                 pixel-centre pinhole camera from look_at_rh(position,
+ *                 target, -up), one PCG stream per pixel seeded x + y*1000
+ *                 through all samples (the seed is not used), unnormalised
+ *                 random_in_unit_sphere, hits only in [1e-3, 1e4], 1e-3
+ *                 offsets, GLSL reflect/refract/mix, a rounding UNORM image,
+ *                 and rd_render_sample rendering every remaining sample at
+ *                 once (vulkan.rs:109-184).  Takes effect at the next
+ *                 new_frame; needs block 256, refill > 0 and no counting;
+ *                 not available in rd_render_frame_noalloc or the batch and
+ *                 server handles. */
 rd_status rd_set_option(rd_renderer *r, const char *key, int64_t value);
 rd_status rd_get_option(const rd_renderer *r, const char *key, int64_t *value);
 
@@ -284,6 +296,14 @@ rd_status rd_get_kernel_times(const rd_renderer *r, rd_kernel_times *out);
  * parameters cgmath asserts against (fov outside (0, 180) degrees, near <= 0,
  * far <= near) or a singular matrix: the reference panics on each. */
 rd_status rd_camera_update_matrices(rd_camera *camera);
+
+/* The camera the Vulkan renderer hands its ray-generation shader
+ * (VulkanRenderer::new_frame, src/renderer/vulkan.rs:251-272): the inverse of
+ * look_at_rh(position, target, -up), column-major, FP32 in cgmath 0.18's
+ * order.  The "semantics" = 1 render option uses it with the scene's stored
+ * inverse projection.  RD_ERR_INVALID_ARGUMENT for a singular view (the
+ * reference unwraps the inverse). */
+rd_status rd_camera_vulkan_inverse_view(const rd_camera *camera, float *out_inverse_view);
 
 /* ---- .rscn I/O (serde layout of the src/scene modules) ----------------- */
 

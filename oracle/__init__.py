@@ -60,8 +60,19 @@ def _bind(L):
     L.or_render_f64.argtypes = [sp, C.c_uint32, C.c_uint32, C.c_uint64, C.c_uint32, C.c_void_p,
                                 C.c_uint32, _P(np.float64), C.c_int]
     L.or_tonemap.argtypes = [_P(np.float32), C.c_uint64, C.c_uint32, _P(np.uint8)]
+    L.or_vk_pcg.argtypes = [C.c_uint32, C.c_uint32, _P(np.uint32)]
+    L.or_vk_camera_rays.argtypes = [sp, _P(np.float32), _P(np.uint32), C.c_uint32, _P(np.float32)]
+    L.or_vk_trace.argtypes = [sp, _P(np.float32), C.c_uint32, _P(np.int32), _P(np.float32),
+                              _P(np.float32), _P(np.uint8)]
+    L.or_vk_scatter.argtypes = [sp, _P(np.float32), _P(np.float32), _P(np.float32)]
+    L.or_vk_world.argtypes = [sp, _P(np.float32), C.c_uint32, _P(np.float32)]
+    L.or_vk_render.argtypes = [sp, _P(np.float32), C.c_uint32, C.c_uint32, C.c_void_p, C.c_uint32,
+                               _P(np.float32), C.c_int]
+    L.or_vk_tonemap.argtypes = [_P(np.float32), C.c_uint64, C.c_uint32, _P(np.uint8)]
     for f in ("or_philox4x32_10", "or_bounce_bits", "or_camera_rays_f32", "or_trace_f32",
-              "or_scatter_f32", "or_world_f32", "or_render_f32", "or_render_f64", "or_tonemap"):
+              "or_scatter_f32", "or_world_f32", "or_render_f32", "or_render_f64", "or_tonemap",
+              "or_vk_pcg", "or_vk_camera_rays", "or_vk_trace", "or_vk_scatter", "or_vk_world",
+              "or_vk_render", "or_vk_tonemap"):
         getattr(L, f).restype = None
     return L
 
@@ -148,3 +159,76 @@ def tonemap(acc, sample_count):
     out = np.zeros(acc.size, np.uint8)
     lib().or_tonemap(acc.reshape(-1), acc.size // 4, sample_count, out)
     return out
+
+
+# ------------------------------------------------ Vulkan-renderer semantics ---
+# vk_inv_view: the inverse of look_at_rh(position, target, -up) (vulkan.rs:252-272),
+# 16 floats column-major; the tests take it from raydar_amd.camera_vulkan_inverse_view.
+
+def vk_pcg(state, n):
+    out = np.zeros(n, np.uint32)
+    lib().or_vk_pcg(state, n, out)
+    return out
+
+
+def vk_camera_rays(scene, vk_inv_view, pixels):
+    pixels = _c(pixels, np.uint32)
+    out = np.zeros((len(pixels), 6), np.float32)
+    s = scene.as_rd()
+    lib().or_vk_camera_rays(C.byref(s), _c(vk_inv_view, np.float32), pixels, len(pixels), out)
+    return out
+
+
+def vk_trace(scene, rays):
+    rays = _c(rays, np.float32).reshape(-1, 6)
+    n = len(rays)
+    ids = np.zeros(n, np.int32)
+    ts = np.zeros(n, np.float32)
+    normals = np.zeros((n, 3), np.float32)
+    front = np.zeros(n, np.uint8)
+    s = scene.as_rd()
+    lib().or_vk_trace(C.byref(s), rays, n, ids, ts, normals, front)
+    return ids, ts, normals, front
+
+
+def vk_scatter(scene, packed, draws):
+    """packed: 14 floats (ray o, d, t, object id bits, atten, light); draws:
+    the uniforms, in order.  Returns 14 floats: next o, d, atten, light,
+    transmission flag, number of uniforms used."""
+    packed = _c(packed, np.float32).reshape(-1)
+    d = np.zeros(16, np.float32)
+    d[:len(draws)] = draws
+    out = np.zeros(14, np.float32)
+    s = scene.as_rd()
+    lib().or_vk_scatter(C.byref(s), packed, d, out)
+    return out
+
+
+def vk_world(scene, dirs):
+    dirs = _c(dirs, np.float32).reshape(-1, 3)
+    out = np.zeros_like(dirs)
+    s = scene.as_rd()
+    lib().or_vk_world(C.byref(s), dirs, len(dirs), out)
+    return out
+
+
+def vk_render(scene, vk_inv_view, spp, max_bounces, pixels=None, threads=None):
+    """Accumulator (n x 4) of all `spp` samples of `pixels` (all when None)."""
+    if threads is None:
+        threads = os.cpu_count() or 1
+    n = scene.width * scene.height if pixels is None else len(pixels)
+    acc = np.zeros((n, 4), np.float32)
+    pix = None if pixels is None else _c(pixels, np.uint32)
+    s = scene.as_rd()
+    lib().or_vk_render(C.byref(s), _c(vk_inv_view, np.float32), spp, max_bounces,
+                       None if pix is None else pix.ctypes.data_as(C.c_void_p),
+                       0 if pix is None else len(pix), acc, threads)
+    return acc
+
+
+def vk_tonemap(acc, sample_count):
+    acc = _c(acc, np.float32)
+    out = np.zeros(acc.size, np.uint8)
+    lib().or_vk_tonemap(acc.reshape(-1), acc.size // 4, sample_count, out)
+    return out
+

@@ -342,9 +342,327 @@ void render(const rd_scene *s, uint32_t spp, uint32_t max_bounces, uint64_t seed
   }
 }
 
+// ------------------------------------------ Vulkan-renderer semantics -------
+// The optional mode that follows the reference's Vulkan backend instead of
+// its CPU renderer (shaders/vulkan/raytrace.rgen, raytrace.rint,
+// raytrace_sphere.rchit, raytrace_triangle.rchit; src/renderer/vulkan.rs).
+// GLSL leaves the rounding of its built-ins to the implementation; this
+// restatement fixes every one as IEEE f32 in the written order:
+//   mat4 * vec4  = ((m[0]*v.x + m[1]*v.y) + m[2]*v.z) + m[3]*v.w per row
+//   normalize(v) = v * (1 / sqrt(dot(v, v))), length(v) = sqrt(dot(v, v))
+//   mix(x, y, a) = x*(1 - a) + y*a, reflect(I, N) = I - (2*dot(N, I))*N
+//   refract(I, N, eta): k = 1 - (eta*eta)*(1 - d*d), d = dot(N, I);
+//       k < 0 -> 0, else eta*I - (eta*d + sqrt(k))*N
+//   pow(x, 2) = x*x, pow(y, 5) = y*((y*y)*(y*y))
+// The hardware traversal is restated as: every object is intersected, a
+// reported t counts if 1e-3 <= t <= 1e4 (the rgen's t_min / t_max), the
+// least such t wins, equal t go to the lower object index.  A cube is the
+// 12-triangle box of vulkan.rs:722-893, taken as its closest surface
+// crossing at or after t_min (the slab test of the CPU renderer), with the
+// face normal of the crossed face (the axis rule of the CPU renderer).
+const float kVkTMin = 1e-3f, kVkTMax = 1e4f;
+
+// raytrace.rgen:52-63: the output is a permutation of the OLD state; the
+// state then steps by the LCG.  rand() = float(word) / float(UINT_MAX).
+inline uint32_t vk_pcg(uint32_t &state) {
+  const uint32_t s = state;
+  state = state * 747796405u + 2891336453u;
+  const uint32_t word = ((s >> ((s >> 28u) + 4u)) ^ s) * 277803737u;
+  return (word >> 22u) ^ word;
+}
+inline float vk_unit(uint32_t word) { return (float)word / 4294967295.0f; }
+
+// A draw source: the PCG stream of one pixel, or a caller's list of uniforms.
+struct VkDraws {
+  uint32_t *state = nullptr;
+  const float *list = nullptr;
+  uint32_t used = 0;
+  float rand() {
+    ++used;
+    return list ? list[used - 1] : vk_unit(vk_pcg(*state));
+  }
+  // raytrace.rgen:65-71: rand_range(-1, 1) per component, not normalised
+  V3<float> rius() {
+    const float x = -1.0f + (1.0f - -1.0f) * rand();
+    const float y = -1.0f + (1.0f - -1.0f) * rand();
+    const float z = -1.0f + (1.0f - -1.0f) * rand();
+    return mk(x, y, z);
+  }
+};
+
+// raytrace.rint:9-28 with the [t_min, t_max] acceptance of the report.
+inline bool vk_hit_sphere(const Ray<float> &r, V3<float> c, float radius, float *t_out) {
+  const float a = dot(r.d, r.d);
+  const float k = dot(r.o, r.d) - dot(r.d, c);
+  const float cc = dot(r.o, r.o) - 2.0f * dot(r.o, c) + dot(c, c) - radius * radius;
+  const float disc = k * k - a * cc;
+  if (!(disc >= 0.0f)) return false;
+  const float s = std::sqrt(disc);
+  const float t1 = (-k - s) / a;
+  const float t2 = (-k + s) / a;
+  float t;
+  if (t1 >= 0.0f) t = t1;        // reported; a t1 below t_min is refused,
+  else if (t2 >= 0.0f) t = t2;   // and t2 is then not reported at all
+  else return false;
+  if (!(t >= kVkTMin && t <= kVkTMax)) return false;
+  *t_out = t;
+  return true;
+}
+
+// The cube's closest surface crossing in [t_min, t_max].
+inline bool vk_hit_cube(const Ray<float> &r, V3<float> c, float side, float *t_out) {
+  V3<float> half = mk(side * 0.5f, side * 0.5f, side * 0.5f);
+  V3<float> mn = c - half, mx = c + half;
+  V3<float> t1 = div_elem(mn - r.o, r.d);
+  V3<float> t2 = div_elem(mx - r.o, r.d);
+  const float tmin = rmax(rmax(rmin(t1.x, t2.x), rmin(t1.y, t2.y)), rmin(t1.z, t2.z));
+  const float tmax = rmin(rmin(rmax(t1.x, t2.x), rmax(t1.y, t2.y)), rmax(t1.z, t2.z));
+  if (!(tmax >= kVkTMin)) return false;
+  if (tmin > tmax) return false;
+  const float t = tmin >= kVkTMin ? tmin : tmax;
+  if (!(t <= kVkTMax)) return false;
+  *t_out = t;
+  return true;
+}
+
+inline int vk_trace(const rd_scene &s, const Ray<float> &r, float *t_best) {
+  int best = -1;
+  float bt = 0.0f;
+  for (uint32_t i = 0; i < s.object_count; ++i) {
+    const rd_object &o = s.objects[i];
+    float t;
+    const bool h = (o.kind == RD_SPHERE) ? vk_hit_sphere(r, load3<float>(o.center), o.size, &t)
+                                         : vk_hit_cube(r, load3<float>(o.center), o.size, &t);
+    if (h && (best < 0 || t < bt)) { best = (int)i; bt = t; }
+  }
+  *t_best = bt;
+  return best;
+}
+
+// raytrace.rgen:83-89: pixel centre, inverse projection, normalised target
+// turned by the inverse view (look_at_rh(position, target, -up) inverted,
+// vulkan.rs:252-272; passed in as vk_inv_view).
+inline void mat_vec(const float *m, const float v[4], float out[4]) {
+  for (int r = 0; r < 4; ++r)
+    out[r] = m[0 + r] * v[0] + m[4 + r] * v[1] + m[8 + r] * v[2] + m[12 + r] * v[3];
+}
+inline Ray<float> vk_camera_ray(const rd_scene &s, const float *vk_inv_view, uint32_t x,
+                                uint32_t y) {
+  const rd_camera &cam = s.camera;
+  const float u = ((float)x + 0.5f) / (float)cam.resolution_x;
+  const float v = ((float)y + 0.5f) / (float)cam.resolution_y;
+  const float dx = u * 2.0f - 1.0f, dy = v * 2.0f - 1.0f;
+  const float zero_w[4] = {0.0f, 0.0f, 0.0f, 1.0f};
+  float o[4], tgt[4], dir[4];
+  mat_vec(vk_inv_view, zero_w, o);
+  const float q[4] = {dx, dy, 1.0f, 1.0f};
+  mat_vec(cam.inverse_proj_matrix, q, tgt);
+  const V3<float> n = normalize(mk(tgt[0], tgt[1], tgt[2]));
+  const float nv[4] = {n.x, n.y, n.z, 0.0f};
+  mat_vec(vk_inv_view, nv, dir);
+  Ray<float> ray;
+  ray.o = mk(o[0], o[1], o[2]);
+  ray.d = mk(dir[0], dir[1], dir[2]);
+  return ray;
+}
+
+// raytrace.rgen:43-48 (a solid world is both colours, vulkan.rs:288-299)
+inline V3<float> vk_world(const rd_world &w, V3<float> d) {
+  V3<float> top, bottom;
+  if (w.kind == RD_WORLD_SOLID_COLOR) {
+    top = bottom = load3<float>(w.solid_color);
+  } else {
+    top = load3<float>(w.top_color);
+    bottom = load3<float>(w.bottom_color);
+  }
+  const V3<float> up = mk(0.0f, 1.0f, 0.0f);
+  const float cs = dot(d, up) / (mag(d) * mag(up));
+  const float a = (cs + 1.0f) * 0.5f;
+  return bottom * (1.0f - a) + top * a;
+}
+
+// raytrace.rgen:106-172: one hit's scatter, drawing in the shader's order.
+inline Scatter<float> vk_scatter(const Ray<float> &ray, const Hit<float> &h, const rd_object &o,
+                                 VkDraws &dr, V3<float> atten, V3<float> light) {
+  const float roughness = o.roughness * o.roughness;
+  V3<float> diffuse = h.n + dr.rius();
+  if (dot(diffuse, h.n) < 0.0f) diffuse = -diffuse;
+  const float d2 = 2.0f * dot(h.n, ray.d);
+  const V3<float> perfect = ray.d - h.n * d2;
+  const V3<float> offset = dr.rius() * roughness;
+  const V3<float> specular = normalize(perfect + offset);
+  const bool trans = dr.rand() < o.transmission;
+  V3<float> dir;
+  if (trans) {
+    float ior = o.ior;
+    if (h.front) ior = 1.0f / ior;
+    const V3<float> rd = normalize(ray.d);
+    const float cos_theta = rmin(dot(rd, -h.n), 1.0f);
+    const float q = (ior - 1.0f) / (ior + 1.0f);
+    const float r0 = q * q;
+    const float y = 1.0f - cos_theta;
+    const float y2 = y * y;
+    const float refl = r0 + (1.0f - r0) * (y * (y2 * y2));
+    const float u = dr.rand();
+    const float c2 = rmin(dot(rd, -h.n), 1.0f);
+    const bool can = ior * std::sqrt(1.0f - c2 * c2) <= 1.0f;
+    if (refl < u && can) {
+      const float dn = dot(h.n, rd);
+      const float k = 1.0f - (ior * ior) * (1.0f - dn * dn);
+      V3<float> refr = mk(0.0f, 0.0f, 0.0f);
+      if (!(k < 0.0f)) refr = rd * ior - h.n * (ior * dn + std::sqrt(k));
+      const V3<float> off2 = dr.rius() * roughness;
+      dir = normalize(refr + off2);
+    } else {
+      dir = specular;
+    }
+  } else if (dr.rand() < o.metallic) {
+    dir = specular;
+  } else if (dr.rand() < roughness) {
+    dir = diffuse;
+  } else {
+    dir = specular;
+  }
+  const V3<float> off = trans ? dir : h.n;
+  Scatter<float> out;
+  out.next.o = h.p + off * 0.001f;
+  out.next.d = dir;
+  if (mag2(out.next.d) < 1e-10f) out.next.d = h.n;
+  out.atten = mul_elem(atten, load3<float>(o.albedo));
+  out.light = light + load3<float>(o.emission_color) * o.emission_strength;
+  out.transmission = trans;
+  return out;
+}
+
+// raytrace.rgen:79-183 for one pixel: every sample in order, one PCG stream
+// seeded x + y*1000 (raytrace.rgen:50) running through all of them.
+inline void vk_pixel(const rd_scene &s, const float *vk_inv_view, uint32_t x, uint32_t y,
+                     uint32_t spp, uint32_t max_bounces, float acc[4]) {
+  uint32_t state = x + y * 1000u;
+  VkDraws dr;
+  dr.state = &state;
+  const Ray<float> cam = vk_camera_ray(s, vk_inv_view, x, y);
+  for (uint32_t k = 0; k < spp; ++k) {
+    Ray<float> ray = cam;
+    V3<float> light = mk(0.0f, 0.0f, 0.0f), atten = mk(1.0f, 1.0f, 1.0f);
+    for (uint32_t b = 0; b < max_bounces; ++b) {
+      float t;
+      const int id = vk_trace(s, ray, &t);
+      if (id < 0) {
+        light = light + mul_elem(vk_world(s.world, ray.d), atten);
+        break;
+      }
+      const rd_object &o = s.objects[id];
+      const Hit<float> h = closest_hit(ray, t, o);
+      const Scatter<float> sc = vk_scatter(ray, h, o, dr, atten, light);
+      ray = sc.next;
+      atten = sc.atten;
+      light = sc.light;
+    }
+    acc[0] = acc[0] + light.x;
+    acc[1] = acc[1] + light.y;
+    acc[2] = acc[2] + light.z;
+    acc[3] = acc[3] + 1.0f;
+  }
+}
+
 }  // namespace
 
 extern "C" {
+
+// ---- Vulkan-renderer semantics (raytrace.rgen); vk_inv_view is the inverse
+// of look_at_rh(position, target, -up) (vulkan.rs:252-272), column-major.
+
+// `n` words of the PCG stream starting from `state`.
+void or_vk_pcg(uint32_t state, uint32_t n, uint32_t *out) {
+  for (uint32_t i = 0; i < n; ++i) out[i] = vk_pcg(state);
+}
+
+void or_vk_camera_rays(const rd_scene *s, const float *vk_inv_view, const uint32_t *pixels,
+                       uint32_t n, float *out) {
+  for (uint32_t i = 0; i < n; ++i) {
+    const uint32_t x = pixels[i] % s->camera.resolution_x, y = pixels[i] / s->camera.resolution_x;
+    const Ray<float> r = vk_camera_ray(*s, vk_inv_view, x, y);
+    float *o = out + 6 * i;
+    o[0] = r.o.x; o[1] = r.o.y; o[2] = r.o.z; o[3] = r.d.x; o[4] = r.d.y; o[5] = r.d.z;
+  }
+}
+
+// Closest hit in [t_min, t_max] for n rays (n x 6): id (-1 = miss), t, normal, front.
+void or_vk_trace(const rd_scene *s, const float *rays, uint32_t n, int32_t *ids, float *ts,
+                 float *normals, uint8_t *front) {
+  for (uint32_t i = 0; i < n; ++i) {
+    const float *q = rays + 6 * i;
+    const Ray<float> r{mk(q[0], q[1], q[2]), mk(q[3], q[4], q[5])};
+    float t = 0.f;
+    const int id = vk_trace(*s, r, &t);
+    ids[i] = id;
+    ts[i] = id >= 0 ? t : 0.f;
+    V3<float> nn = mk(0.f, 0.f, 0.f);
+    uint8_t f = 0;
+    if (id >= 0) {
+      const Hit<float> h = closest_hit(r, t, s->objects[id]);
+      nn = h.n;
+      f = h.front;
+    }
+    normals[3 * i] = nn.x; normals[3 * i + 1] = nn.y; normals[3 * i + 2] = nn.z;
+    front[i] = f;
+  }
+}
+
+// One scatter with caller-chosen uniforms: in = ray o(3) d(3), t, object id
+// (float bits), atten(3), light(3); draws: up to 11 uniforms.  out = next
+// o(3) d(3), atten(3), light(3), transmission flag, uniforms used.
+void or_vk_scatter(const rd_scene *s, const float *in, const float *draws, float *out) {
+  const Ray<float> r{mk(in[0], in[1], in[2]), mk(in[3], in[4], in[5])};
+  int32_t id;
+  std::memcpy(&id, &in[7], 4);
+  const rd_object &o = s->objects[id];
+  const Hit<float> h = closest_hit(r, in[6], o);
+  VkDraws dr;
+  dr.list = draws;
+  const Scatter<float> sc = vk_scatter(r, h, o, dr, mk(in[8], in[9], in[10]),
+                                       mk(in[11], in[12], in[13]));
+  out[0] = sc.next.o.x; out[1] = sc.next.o.y; out[2] = sc.next.o.z;
+  out[3] = sc.next.d.x; out[4] = sc.next.d.y; out[5] = sc.next.d.z;
+  out[6] = sc.atten.x; out[7] = sc.atten.y; out[8] = sc.atten.z;
+  out[9] = sc.light.x; out[10] = sc.light.y; out[11] = sc.light.z;
+  out[12] = sc.transmission ? 1.f : 0.f;
+  out[13] = (float)dr.used;
+}
+
+void or_vk_world(const rd_scene *s, const float *dirs, uint32_t n, float *out) {
+  for (uint32_t i = 0; i < n; ++i) {
+    const V3<float> c = vk_world(s->world, mk(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]));
+    out[3 * i] = c.x; out[3 * i + 1] = c.y; out[3 * i + 2] = c.z;
+  }
+}
+
+// The listed pixels (or all, pixels == NULL), all `spp` samples each, summed
+// into acc (n x 4) in sample order.
+void or_vk_render(const rd_scene *s, const float *vk_inv_view, uint32_t spp, uint32_t max_bounces,
+                  const uint32_t *pixels, uint32_t n_pixels, float *acc, int threads) {
+  const uint32_t W = s->camera.resolution_x;
+  const int64_t n = pixels ? (int64_t)n_pixels : (int64_t)W * s->camera.resolution_y;
+#pragma omp parallel for schedule(dynamic, 16) num_threads(threads > 0 ? threads : 1)
+  for (int64_t i = 0; i < n; ++i) {
+    const uint32_t p = pixels ? pixels[i] : (uint32_t)i;
+    vk_pixel(*s, vk_inv_view, p % W, p / W, spp, max_bounces, acc + 4 * i);
+  }
+}
+
+// imageStore of accumulated / spp into R8G8B8A8_UNORM (raytrace.rgen:186,
+// vulkan.rs:196): clamp to [0, 1], scale by 255, round to nearest (ties to
+// even); NaN -> 0.
+void or_vk_tonemap(const float *acc, uint64_t n_pixels, uint32_t sample_count, uint8_t *rgba) {
+  const float spp = (float)sample_count;
+  for (uint64_t i = 0; i < 4 * n_pixels; ++i) {
+    float v = acc[i] / spp;
+    if (std::isnan(v)) { rgba[i] = 0; continue; }
+    v = v < 0.f ? 0.f : (v > 1.f ? 1.f : v);
+    rgba[i] = (uint8_t)std::nearbyint(v * 255.0f);
+  }
+}
 
 void or_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
   philox4x32_10(ctr, key, out);

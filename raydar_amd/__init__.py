@@ -60,6 +60,7 @@ def lib():
         "rd_reset_counters": (C.c_int, [R]),
         "rd_get_kernel_times": (C.c_int, [R, P(abi.RdKernelTimes)]),
         "rd_camera_update_matrices": (C.c_int, [P(abi.RdCamera)]),
+        "rd_camera_vulkan_inverse_view": (C.c_int, [P(abi.RdCamera), C.c_void_p]),
         "rd_scene_load_rscn": (C.c_int, [C.c_char_p, P(sp)]),
         "rd_scene_parse_rscn": (C.c_int, [C.c_char_p, C.c_size_t, P(sp)]),
         "rd_scene_save_rscn": (C.c_int, [sp, C.c_char_p]),
@@ -250,6 +251,15 @@ def camera_update_matrices(camera):
     out = dict(camera)
     for k in ("view_matrix", "proj_matrix", "inverse_view_matrix", "inverse_proj_matrix"):
         out[k] = list(getattr(s, k))
+    return out
+
+
+def camera_vulkan_inverse_view(camera):
+    """The Vulkan renderer's inverse view (vulkan.rs:251-272) through the C
+    ABI: 16 floats, column-major."""
+    s = Scene(camera, None, np.zeros(0, abi.OBJECT_DTYPE)).as_rd_camera()
+    out = np.zeros(16, np.float32)
+    check(lib().rd_camera_vulkan_inverse_view(C.byref(s), out.ctypes.data_as(C.c_void_p)))
     return out
 
 

@@ -132,8 +132,10 @@ struct rd_renderer {
   bool job_uploaded = false;
   float *acc = nullptr;
   uchar4 *rgba_d = nullptr;
+  uint32_t *vk_rng = nullptr;  // Vulkan semantics: per-pixel PCG state
   uint32_t W = 0, H = 0;
   bool has_frame = false;
+  int frame_vk = 0;  // the semantics the current frame was set up with
   rdh::Timer frame_timer, sample_timer, prepare_timer, render_timer;
 
   rd_status fail(rd_status st, const std::string &msg) {
@@ -183,8 +185,10 @@ rd_status snapshot(rd_renderer *r, const rd_scene *s, bool zero_acc) {
   if (W != r->W || H != r->H || !r->acc) {
     r->eng.dfree(r->acc);
     r->eng.dfree(r->rgba_d);
+    r->eng.dfree(r->vk_rng);
     r->acc = nullptr;
     r->rgba_d = nullptr;
+    r->vk_rng = nullptr;
     if (!r->eng.check(r->eng.dmalloc((void **)&r->acc, 16ull * W * H), "hipMalloc(accumulator)", &err) ||
         !r->eng.check(r->eng.dmalloc((void **)&r->rgba_d, 4ull * W * H), "hipMalloc(image)", &err))
       return device_fail(r, err);
@@ -196,6 +200,16 @@ rd_status snapshot(rd_renderer *r, const rd_scene *s, bool zero_acc) {
                                 "zero accumulator", &err))
     return device_fail(r, err);
   rdh::fill_job(*s, r->scene, r->acc, r->cfg.max_bounces, r->seed, &r->job_h);
+  r->frame_vk = r->eng.opt.vk;
+  if (r->frame_vk) {
+    // the Vulkan renderer's camera (vulkan.rs:251-272) and a PCG state per pixel
+    if (rd_camera_vulkan_inverse_view(&s->camera, r->job_h.inv_view) != RD_OK)
+      return r->fail(RD_ERR_INVALID_ARGUMENT, "singular Vulkan view matrix");
+    if (!r->vk_rng && !r->eng.check(r->eng.dmalloc((void **)&r->vk_rng, 4ull * W * H),
+                                    "hipMalloc(PCG states)", &err))
+      return device_fail(r, err);
+    r->job_h.vk_rng = r->vk_rng;
+  }
   if (!r->job_d && !r->eng.check(hipMalloc(&r->job_d, sizeof(rdh::DevJob)), "hipMalloc(job)", &err))
     return device_fail(r, err);
   *r->job_pin = r->job_h;  // no upload from it is in flight: every call ends synchronised
@@ -229,7 +243,17 @@ rd_status refresh_job(rd_renderer *r) {
 // run as one pass, so the sample timer reports their average, as the Vulkan
 // renderer does for its all-samples dispatch (vulkan.rs:175-177); with k = 1
 // that is the last sample's time, as CpuRenderer reports (cpu.rs:196, 218).
+//
+// Vulkan semantics: a pixel's samples share one PCG stream whose position
+// after a sample depends on that sample's draws, so each pass holds one
+// sample (all pixels), and the next pass continues every pixel's stream.
 rd_status render_samples(rd_renderer *r, uint32_t k) {
+  const rdh::EngineOptions &o = r->eng.opt;
+  if (o.vk != r->frame_vk)
+    return r->fail(RD_ERR_INVALID_ARGUMENT, "\"semantics\" changed after new_frame: start a new frame");
+  if (o.vk && (o.block != 256 || o.refill <= 0 || o.count))
+    return r->fail(RD_ERR_INVALID_ARGUMENT,
+                   "\"semantics\" = 1 runs with block 256, refill > 0 and no counting");
   r->prepare_timer.end_if_not_ended();
   r->render_timer.start_if_not_started();
   r->sample_timer.start();
@@ -242,6 +266,7 @@ rd_status render_samples(rd_renderer *r, uint32_t k) {
   while (done < k) {
     uint64_t fit = r->eng.opt.l_capacity / npix;
     if (fit == 0) return r->fail(RD_ERR_INVALID_ARGUMENT, "image larger than the radiance arena");
+    if (o.vk) fit = 1;
     uint32_t spp = (uint32_t)std::min<uint64_t>({(uint64_t)(k - done), fit, 0xffffffffull / npix});
     rdh::DevTicket t{};
     t.job = 0;
@@ -268,7 +293,7 @@ rd_status tonemap_out(rd_renderer *r, uint8_t *rgba, size_t cap) {
   const int grid = (int)std::min<uint64_t>((npix + 255) / 256, 8192);
   if (!r->eng.check(rdh::launch_tonemap(r->job_d, (uint32_t)npix, (float)r->sample_count,
                                         r->rgba_d, r->eng.counters, r->eng.opt.count != 0, grid,
-                                        r->eng.stream),
+                                        r->eng.stream, r->frame_vk != 0),
                     "tonemap", &err) ||
       !r->eng.check(hipMemcpyAsync(rgba, r->rgba_d, 4 * npix, hipMemcpyDeviceToHost, r->eng.stream),
                     "read image", &err) ||
@@ -301,6 +326,7 @@ bool option_value(const rd_renderer *r, const std::string &k, int64_t *v) {
   else if (k == "bvh_nodes") *v = r->scene.n_nodes;
   else if (k == "iterations") *v = (int64_t)r->eng.iterations;
   else if (k == "num_cus") *v = r->eng.num_cus;
+  else if (k == "semantics") *v = o.vk;
   else return false;
   return true;
 }
@@ -345,6 +371,7 @@ void rd_destroy(rd_renderer *r) {
   r->eng.free_scene(&r->scene);
   r->eng.dfree(r->acc);
   r->eng.dfree(r->rgba_d);
+  r->eng.dfree(r->vk_rng);
   if (r->job_d) (void)hipFree(r->job_d);
   if (r->job_pin) (void)hipHostFree(r->job_pin);
   delete r;
@@ -386,7 +413,9 @@ rd_status rd_render_sample(rd_renderer *r, const rd_scene *scene, uint8_t *rgba,
   }
   if (!rgba || cap < 4ull * r->W * r->H)
     return r->fail(RD_ERR_BUFFER_TOO_SMALL, "image buffer smaller than W*H*4 bytes");
-  const uint32_t k = std::min(r->samples_per_pass, r->cfg.max_sample_count - r->sample_count);
+  // the Vulkan renderer renders every sample in one dispatch (vulkan.rs:109-184)
+  const uint32_t k = r->frame_vk ? r->cfg.max_sample_count - r->sample_count
+                                 : std::min(r->samples_per_pass, r->cfg.max_sample_count - r->sample_count);
   rd_status st = render_samples(r, k);
   if (st != RD_OK) return st;
   st = tonemap_out(r, rgba, cap);
@@ -419,6 +448,8 @@ rd_status rd_render_frame_noalloc(rd_renderer *r, const rd_scene *scene, uint8_t
   const uint32_t W = scene->camera.resolution_x, H = scene->camera.resolution_y;
   const uint64_t npix = (uint64_t)W * H;
   if (!rgba || cap < 4 * npix) return r->fail(RD_ERR_BUFFER_TOO_SMALL, "image buffer smaller than W*H*4 bytes");
+  if (r->eng.opt.vk)
+    return r->fail(RD_ERR_UNSUPPORTED, "\"semantics\" = 1 is not available without allocation");
   const int dev = r->eng.device;
   uint8_t *base = static_cast<uint8_t *>(scratch_addr(dev));
   if (!base) return r->fail(RD_ERR_DEVICE, "static scratch not set up on this device");
@@ -669,6 +700,9 @@ rd_status rd_set_option(rd_renderer *r, const char *key, int64_t v) {
     o.sync_every = (int)v;
   } else if (k == "samples_per_pass") {
     return rd_set_samples_per_pass(r, (uint32_t)std::max<int64_t>(0, std::min<int64_t>(v, 0xffffffffll)));
+  } else if (k == "semantics") {
+    if (v != 0 && v != 1) return r->fail(RD_ERR_INVALID_ARGUMENT, "semantics must be 0 (CPU renderer) or 1 (Vulkan renderer)");
+    o.vk = (int)v;
   } else {
     return r->fail(RD_ERR_INVALID_ARGUMENT, "unknown option " + k);
   }

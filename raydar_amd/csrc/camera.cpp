@@ -42,8 +42,7 @@ V3 cross(V3 a, V3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x
 V3 load(const float *p) { return {p[0], p[1], p[2]}; }
 
 // m[c * 4 + r]: column c, row r (cgmath's m[c][r])
-void look_at_lh(V3 eye, V3 center, V3 up, float *m) {
-  const V3 dir = neg(sub(center, eye));
+void look_to_rh(V3 eye, V3 dir, V3 up, float *m) {
   const V3 f = normalize(dir);
   const V3 s = normalize(cross(f, up));
   const V3 u = cross(s, f);
@@ -51,6 +50,8 @@ void look_at_lh(V3 eye, V3 center, V3 up, float *m) {
                        s.z, u.z, -f.z, 0.0f, -dot(eye, s), -dot(eye, u), dot(eye, f), 1.0f};
   std::memcpy(m, v, sizeof(v));
 }
+void look_at_lh(V3 eye, V3 center, V3 up, float *m) { look_to_rh(eye, neg(sub(center, eye)), up, m); }
+void look_at_rh(V3 eye, V3 center, V3 up, float *m) { look_to_rh(eye, sub(center, eye), up, m); }
 
 void perspective(float fovy_deg, float aspect, float n, float f, float *m) {
   const float rad = fovy_deg * (float)(M_PI / 180.0);
@@ -154,5 +155,17 @@ extern "C" rd_status rd_camera_update_matrices(rd_camera *c) {
   std::memcpy(c->proj_matrix, proj, sizeof(proj));
   std::memcpy(c->inverse_view_matrix, iv, sizeof(iv));
   std::memcpy(c->inverse_proj_matrix, ip, sizeof(ip));
+  return RD_OK;
+}
+
+// The Vulkan renderer's camera (vulkan.rs:251-272): view = look_at_rh(position,
+// target, up * -1) ("left-handed to right-handed and flip Y"), handed to the
+// ray-generation shader inverted.  cgmath's `up * -1.0` negates exactly.
+extern "C" rd_status rd_camera_vulkan_inverse_view(const rd_camera *c, float *out_inverse_view) {
+  if (!c || !out_inverse_view) return RD_ERR_INVALID_ARGUMENT;
+  float view[16], iv[16];
+  look_at_rh(load(c->position), load(c->target), neg(load(c->up)), view);
+  if (!invert(view, iv)) return RD_ERR_INVALID_ARGUMENT;  // vulkan.rs:271 unwraps
+  std::memcpy(out_inverse_view, iv, sizeof(iv));
   return RD_OK;
 }

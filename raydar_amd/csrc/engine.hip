@@ -335,6 +335,7 @@ IterArgs Engine::iter_args(const DevJob *d_jobs) const {
   a.lds_obj_bytes = 0;
   a.bvh4 = 0;
   a.leaf1 = 0;
+  a.vk = opt.vk;
   a.slots = (uint32_t)kSegments * seg_cap;
   a.ticket_cap = ticket_alloc;
   a.job_cap = 1;
@@ -567,7 +568,8 @@ bool Engine::run_tickets(const DevJob *d_jobs, std::vector<DevTicket> tickets,
         // the traversal can need.
         const DevJob &j = host_jobs[0];
         a.bvh4 = opt.bvh4 && j.n_nodes4 > 0 && j.stack4 + 3 <= (uint32_t)kStackDepth ? 1 : 0;
-        a.leaf1 = a.bvh4 && j.max_leaf == 1 ? 1 : 0;
+        // (the Vulkan-semantics kernels have no one-primitive-leaf variant)
+        a.leaf1 = a.bvh4 && j.max_leaf == 1 && !a.vk ? 1 : 0;
         const uint64_t nb = a.bvh4 ? (uint64_t)sizeof(DevNode4) * j.n_nodes4 : (uint64_t)sizeof(DevNode) * j.n_nodes;
         const uint64_t bytes = nb + (uint64_t)sizeof(DevPrim) * j.n_prims;
         const uint32_t rows = a.bvh4 ? j.stack4 + 3 : j.depth + 1;

@@ -84,6 +84,8 @@ struct EngineOptions {
   int pipes = 0;                  // one-scene passes: independent streams the
                                   // samples are divided over (1..4; 0 = 2 for
                                   // LDS-resident scenes, else 3)
+  int vk = 0;                     // 1: the Vulkan renderer's semantics (per-scene
+                                  // renderer only)
 };
 
 // Device buffers an engine borrows instead of allocating (the allocation-free

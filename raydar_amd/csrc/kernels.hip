@@ -290,7 +290,10 @@ template <bool SINGLE>
 __device__ __forceinline__ uint32_t ticket_of(uint32_t tb) { return SINGLE ? 0u : (tb >> 8); }
 
 // ------------------------------------------------------------ generate ----
-template <bool COUNT, bool SINGLE>
+// VK (Vulkan semantics, one sample per pass): the pinhole camera of
+// raytrace.rgen:83-89; the pass of sample 0 also seeds each pixel's PCG
+// state with x + y * 1000 (raytrace.rgen:50).
+template <bool COUNT, bool SINGLE, bool VK = false>
 __global__ void k_generate(IterArgs a, int p) {
   __shared__ SegPlan pl;
   make_plan(a, p, pl);
@@ -319,7 +322,13 @@ __global__ void k_generate(IterArgs a, int p) {
     const uint32_t npix = J.width * J.height;
     const uint32_t pixel = pidx % npix;
     F3 o, d;
-    camera_ray(J, pixel % J.width, pixel / J.width, &o, &d);
+    if (VK) {
+      const uint32_t x = pixel % J.width, y = pixel / J.width;
+      camera_ray_vk(J, x, y, &o, &d);
+      if (T.sample_base == 0) J.vk_rng[chk(pixel, npix, K_RADIANCE, a.ctl)] = x + y * 1000u;
+    } else {
+      camera_ray(J, pixel % J.width, pixel / J.width, &o, &d);
+    }
     q.ox[k] = o.x; q.oy[k] = o.y; q.oz[k] = o.z;
     q.dx[k] = d.x; q.dy[k] = d.y; q.dz[k] = d.z;
     // attenuation and light of a camera path are (1,1,1) and 0 (cpu.rs:253-254):
@@ -406,9 +415,14 @@ constexpr int lanes_min_waves(bool single, bool w4) { return !single && w4 ? 7 :
 // LEAF1 (single 4-wide jobs whose leaves hold one primitive each, IterArgs::
 // leaf1): a hit leaf child contributes exactly one primitive, so the leaf
 // counts are the hit flags and each trip's primitive is its child's first.
-template <int BLOCK, bool COUNT, bool SINGLE, class StackT, bool LDS, bool W4, bool LEAF1 = false>
+//
+// VK (Vulkan semantics): hits count only inside [t_min, t_max]
+// (test_prim_vk), and the cull starts at t_max instead of infinity.
+template <int BLOCK, bool COUNT, bool SINGLE, class StackT, bool LDS, bool W4, bool LEAF1 = false,
+          bool VK = false>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(lanes_min_waves(SINGLE, W4), 8)))
 void k_extend_lanes(IterArgs a, int p) {
+  const float cull0 = VK ? cull_limit(kVkTMax, 0u) : INFINITY;
   __shared__ StackT static_stack[LDS ? 1 : kStackDepth * BLOCK];
   extern __shared__ __align__(16) unsigned char dynamic_lds[];
   __shared__ SegPlan pl;
@@ -500,7 +514,8 @@ void k_extend_lanes(IterArgs a, int p) {
               if (fbits(pr.b[3]) & kTagCube) tc.cube++;
               else tc.sphere++;
             }
-            test_prim(r, pr, &bt, &bid);
+            if (VK) test_prim_vk(r, pr, &bt, &bid);
+            else test_prim(r, pr, &bt, &bid);
           }
           a.hit_t[k] = bid == kNoHit ? 0.0f : bt;
           a.hit_id[k] = bid;
@@ -515,7 +530,7 @@ void k_extend_lanes(IterArgs a, int p) {
           if (W4) nf = slab_nf(s);
           node = 0;
           sp = 0;
-          tcull = INFINITY;  // cull_limit of "no hit yet"
+          tcull = cull0;  // cull_limit of "no hit yet"
           active = true;
         }
       }
@@ -634,9 +649,10 @@ void k_extend_lanes(IterArgs a, int p) {
               if (fbits(pr.b[3]) & kTagCube) tc.cube++;
               else tc.sphere++;
             }
-            test_prim(r, pr, &bt, &bid);
+            if (VK) test_prim_vk(r, pr, &bt, &bid);
+            else test_prim(r, pr, &bt, &bid);
           }
-          tcull = cull_limit(bt, bid);
+          tcull = VK && bid == kNoHit ? cull0 : cull_limit(bt, bid);
         }
         if (done) {
           a.hit_t[k] = bid == kNoHit ? 0.0f : bt;
@@ -689,9 +705,11 @@ void k_extend_lanes(IterArgs a, int p) {
               if (fbits(pr.b[3]) & kTagCube) tc.cube++;
               else tc.sphere++;
             }
-            test_prim(r, pr, &bt, &bid);
+            if (VK) test_prim_vk(r, pr, &bt, &bid);
+            else test_prim(r, pr, &bt, &bid);
           }
-          tcull = cull_limit(bt, bid);  // changes only when a primitive is tested
+          // changes only when a primitive is tested
+          tcull = VK && bid == kNoHit ? cull0 : cull_limit(bt, bid);
         }
         // next node without branches: the far child is written above the
         // stack top every step (kept only when both children are entered;
@@ -744,7 +762,10 @@ void k_extend_lanes(IterArgs a, int p) {
 // variant (94 VGPRs) would spill 64 B, so it keeps the compiler's choice.
 constexpr int shade_min_waves(bool single) { return single ? 7 : 1; }
 
-template <int BLOCK, bool COUNT, bool SINGLE>
+// VK (Vulkan semantics, single job, one sample per pass): the scatter and
+// sky of raytrace.rgen, drawing from the pixel's PCG state, which the path
+// (its pixel's only one in the pass) reads and writes back at every hit.
+template <int BLOCK, bool COUNT, bool SINGLE, bool VK = false>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(shade_min_waves(SINGLE), 8)))
 void k_shade(IterArgs a, int p) {
   constexpr int kWaves = BLOCK / 64;
@@ -807,9 +828,17 @@ void k_shade(IterArgs a, int p) {
                                      : load_object(J.objects, oid);
         const HitRec h = closest_hit(o, d, a.hit_t[k], ob);
         const uint32_t npix = J.width * J.height;
-        BounceDraws dr;
-        dr.init(J.seed, pidx % npix, T.sample_base + pidx / npix, bounce);
-        const ScatterOut so = scatter(d, h, ob, dr, at, li);
+        ScatterOut so;
+        if (VK) {
+          uint32_t *rng = &J.vk_rng[chk(pidx % npix, npix, K_RADIANCE, a.ctl)];
+          uint32_t st = *rng;
+          so = scatter_vk(d, h, ob, st, at, li);
+          *rng = st;
+        } else {
+          BounceDraws dr;
+          dr.init(J.seed, pidx % npix, T.sample_base + pidx / npix, bounce);
+          so = scatter(d, h, ob, dr, at, li);
+        }
         if (COUNT) {
           if (fbits(ob.m2[2]) == RD_SPHERE) c_sph++;
           else c_cube++;
@@ -824,7 +853,8 @@ void k_shade(IterArgs a, int p) {
           finished = true;  // bounce budget spent: no sky term (cpu.rs:256)
         }
       } else {
-        nl = add(li, mul(world_sample(J, d), at));  // cpu.rs:334-337
+        // cpu.rs:334-337 (raytrace.rgen:178)
+        nl = add(li, mul(VK ? world_sample_vk(J, d) : world_sample(J, d), at));
         finished = true;
         if (COUNT) c_miss++;
       }
@@ -916,7 +946,8 @@ __global__ void k_accumulate(const DevJob *job, DevTicket T, const float4 *L,
 }
 
 // ------------------------------------------------------------- tonemap ----
-template <bool COUNT>
+// VK: the Vulkan renderer's UNORM store (rounds) instead of print_frame_buffer.
+template <bool COUNT, bool VK = false>
 __global__ void k_tonemap(const DevJob *job, uint32_t n, float spp, uchar4 *out,
                           DevCounters *counters) {
   const float4 *acc = reinterpret_cast<const float4 *>(job->acc);
@@ -924,8 +955,12 @@ __global__ void k_tonemap(const DevJob *job, uint32_t n, float spp, uchar4 *out,
   for (uint32_t px = blockIdx.x * blockDim.x + threadIdx.x; px < n;
        px += gridDim.x * blockDim.x) {
     const float4 v = acc[px];
-    out[px] = make_uchar4(tonemap_channel(v.x, spp), tonemap_channel(v.y, spp),
-                          tonemap_channel(v.z, spp), tonemap_channel(v.w, spp));
+    if (VK)
+      out[px] = make_uchar4(tonemap_channel_vk(v.x, spp), tonemap_channel_vk(v.y, spp),
+                            tonemap_channel_vk(v.z, spp), tonemap_channel_vk(v.w, spp));
+    else
+      out[px] = make_uchar4(tonemap_channel(v.x, spp), tonemap_channel(v.y, spp),
+                            tonemap_channel(v.z, spp), tonemap_channel(v.w, spp));
     ++done;
   }
   if (COUNT && done) atomicAdd(&counters->v[C_TONEMAP], (unsigned long long)done);
@@ -1097,6 +1132,19 @@ static const void *shade_fn(int block) {
   if (block == 256) return (const void *)k_shade<256, COUNT, SINGLE>;
   return nullptr;
 }
+// The Vulkan-semantics kernels: single job, 256 threads, persistent lanes,
+// no counting build.
+static const void *vk_kernel_fn(int which, bool wide, bool lds, bool w4) {
+  if (which == 0) return (const void *)k_generate<false, true, true>;
+  if (which == 2) return (const void *)k_shade<256, false, true, true>;
+  if (w4 && !wide)
+    return lds ? (const void *)k_extend_lanes<256, false, true, uint16_t, true, true, false, true>
+               : (const void *)k_extend_lanes<256, false, true, uint16_t, false, true, false, true>;
+  if (lds && !wide)
+    return (const void *)k_extend_lanes<256, false, true, uint16_t, true, false, false, true>;
+  return wide ? (const void *)k_extend_lanes<256, false, true, int32_t, false, false, false, true>
+              : (const void *)k_extend_lanes<256, false, true, uint16_t, false, false, false, true>;
+}
 static const void *kernel_fn(int which, int block, bool count, bool single, bool wide,
                              bool lanes = false, bool lds = false, bool w4 = false,
                              bool leaf1 = false) {
@@ -1136,8 +1184,15 @@ uint32_t shade_dynamic_lds(const IterArgs &a) { return a.single_job ? a.lds_obj_
 static hipError_t launch_stage(int which, const IterArgs &a, int parity, const LaunchShape &s,
                                bool count, hipStream_t st) {
   const uint32_t dyn = which == 1 ? extend_dynamic_lds(a, s.block) : which == 2 ? shade_dynamic_lds(a) : 0;
-  const void *fn = kernel_fn(which, s.block, count, a.single_job != 0, a.wide_stack != 0,
-                             a.refill > 0, dyn > 0, a.bvh4 != 0, a.leaf1 != 0);
+  const void *fn;
+  if (a.vk) {
+    // the host admits Vulkan semantics only for these launches
+    if (count || s.block != 256 || !a.single_job || a.refill <= 0) return hipErrorInvalidValue;
+    fn = vk_kernel_fn(which, a.wide_stack != 0, dyn > 0, a.bvh4 != 0);
+  } else {
+    fn = kernel_fn(which, s.block, count, a.single_job != 0, a.wide_stack != 0, a.refill > 0,
+                   dyn > 0, a.bvh4 != 0, a.leaf1 != 0);
+  }
   if (!fn) return hipErrorInvalidValue;
   IterArgs args = a;
   int par = parity;
@@ -1168,8 +1223,9 @@ hipError_t launch_accumulate(const DevJob *job, const DevTicket &t, const float4
 }
 
 hipError_t launch_tonemap(const DevJob *job, uint32_t n_pixels, float spp, uchar4 *out,
-                          DevCounters *counters, bool count, int grid, hipStream_t st) {
-  if (count) hipLaunchKernelGGL(k_tonemap<true>, dim3(grid), dim3(256), 0, st, job, n_pixels, spp, out, counters);
+                          DevCounters *counters, bool count, int grid, hipStream_t st, bool vk) {
+  if (vk) hipLaunchKernelGGL((k_tonemap<false, true>), dim3(grid), dim3(256), 0, st, job, n_pixels, spp, out, counters);
+  else if (count) hipLaunchKernelGGL(k_tonemap<true>, dim3(grid), dim3(256), 0, st, job, n_pixels, spp, out, counters);
   else hipLaunchKernelGGL(k_tonemap<false>, dim3(grid), dim3(256), 0, st, job, n_pixels, spp, out, counters);
   return hipGetLastError();
 }

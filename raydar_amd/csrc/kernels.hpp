@@ -47,6 +47,8 @@ struct IterArgs {
                                     // each shade workgroup copies into LDS
   int bvh4;                         // single job: traverse DevJob::nodes4
   int leaf1;                        // single 4-wide job whose leaves hold one primitive each
+  int vk;                           // 1: the Vulkan renderer's semantics (single job,
+                                    // block 256, one sample per pass, no counting)
   // Buffer sizes, read only by bounds-checked builds (RDH_CHECKED):
   uint32_t slots;                   // entries of each queue array and hit array
   uint32_t ticket_cap;              // ticket table entries
@@ -72,8 +74,10 @@ hipError_t launch_shade(const IterArgs &a, int parity, const LaunchShape &s, boo
                         hipStream_t st);
 hipError_t launch_accumulate(const DevJob *job, const DevTicket &t, const float4 *L,
                              DevCounters *counters, bool count, int grid, hipStream_t st);
+// vk: the Vulkan renderer's rounding UNORM store instead of print_frame_buffer
 hipError_t launch_tonemap(const DevJob *job, uint32_t n_pixels, float spp, uchar4 *out,
-                          DevCounters *counters, bool count, int grid, hipStream_t st);
+                          DevCounters *counters, bool count, int grid, hipStream_t st,
+                          bool vk = false);
 // Workgroups of kernel `which` (0 generate, 1 extend, 2 shade) that fit on one CU.
 int occupancy_blocks(int which, int block, bool lanes = false, uint32_t lds_bvh_bytes = 0,
                      uint32_t dynamic_lds = 0, bool w4 = false, bool single = true);

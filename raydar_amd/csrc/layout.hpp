@@ -106,6 +106,7 @@ struct DevJob {
   uint32_t stack4;          // stack entries a 4-wide traversal can hold
   uint32_t max_leaf;        // primitives in the largest leaf (1: one per leaf)
   const DevNode4 *nodes4;
+  uint32_t *vk_rng;         // Vulkan semantics: each pixel's PCG state between passes
 };
 
 struct DevTicket {

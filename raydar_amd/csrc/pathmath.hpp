@@ -439,4 +439,170 @@ RD_HD uint8_t tonemap_channel(float acc, float spp) {
   return (uint8_t)(v * 255.0f);
 }
 
+// ======================================= Vulkan-renderer semantics ========
+// The optional mode (render option "semantics" = 1) that follows the
+// reference's Vulkan backend: shaders/vulkan/raytrace.rgen, raytrace.rint,
+// raytrace_sphere.rchit, raytrace_triangle.rchit and the camera, cube and
+// output set-up of src/renderer/vulkan.rs.  GLSL leaves its built-ins'
+// rounding to the implementation; here each is IEEE f32 in the written order
+// (normalize(v) = v * (1/sqrt(dot(v,v))), mix(x,y,a) = x*(1-a) + y*a,
+// pow(x,2) = x*x, pow(y,5) = y*((y*y)*(y*y))).  A cube is the box of its 12
+// triangles (vulkan.rs:722-893), hit at its closest surface crossing inside
+// the ray's [t_min, t_max], with the crossed face's normal.
+constexpr float kVkTMin = 1e-3f;  // raytrace.rgen:103
+constexpr float kVkTMax = 1e4f;   // raytrace.rgen:105
+
+// raytrace.rgen:52-57: returns the RXS-M-XS output of the current state and
+// steps the state by the LCG.
+RD_HD uint32_t vk_pcg(uint32_t &state) {
+  const uint32_t s = state;
+  state = s * 747796405u + 2891336453u;
+  const uint32_t w = ((s >> ((s >> 28u) + 4u)) ^ s) * 277803737u;
+  return (w >> 22u) ^ w;
+}
+// raytrace.rgen:59-67: float(word) / UINT_MAX (UINT_MAX rounds to 2^32 in
+// f32, so U[0,1]); rand_range(-1, 1) = -1 + 2u.
+RD_HD float vk_rand(uint32_t &state) { return (float)vk_pcg(state) / 4294967295.0f; }
+RD_HD float vk_rand_signed(uint32_t &state) { return -1.0f + 2.0f * vk_rand(state); }
+// raytrace.rgen:69-71, not normalised; x, y, z drawn in that order
+RD_HD F3 vk_rius(uint32_t &state) {
+  const float x = vk_rand_signed(state);
+  const float y = vk_rand_signed(state);
+  const float z = vk_rand_signed(state);
+  return f3(x, y, z);
+}
+
+// raytrace.rint:16-27 and the report's [t_min, t_max] test: t1 is reported
+// when t1 >= 0 (and refused below t_min, t2 then not tried), else t2.
+RD_HD bool hit_sphere_vk(const RayPre &r, const float *pa, const float *pb, float *t) {
+  const float dC = r.dx * pa[0] + r.dy * pa[1] + r.dz * pa[2];
+  const float oC = r.ox * pa[0] + r.oy * pa[1] + r.oz * pa[2];
+  const float k = r.od - dC;
+  const float c = r.oo - 2.0f * oC + pa[3] - pb[0];
+  const float disc = k * k - r.a * c;
+  if (!(disc >= 0.0f)) return false;
+  const float s = sqrtf(disc);
+  float tt = (-k - s) / r.a;
+  if (!(tt >= 0.0f)) {
+    tt = (-k + s) / r.a;
+    if (!(tt >= 0.0f)) return false;
+  }
+  *t = tt;
+  return tt >= kVkTMin && tt <= kVkTMax;
+}
+
+// The box's first crossing at or after t_min: its entry, or its exit for a
+// ray that enters before t_min.
+RD_HD bool hit_cube_vk(const RayPre &r, const float *pa, const float *pb, float *t) {
+  const float t1x = (pa[0] - r.ox) / r.dx, t2x = (pb[0] - r.ox) / r.dx;
+  const float t1y = (pa[1] - r.oy) / r.dy, t2y = (pb[1] - r.oy) / r.dy;
+  const float t1z = (pa[2] - r.oz) / r.dz, t2z = (pb[2] - r.oz) / r.dz;
+  const float tmin = fmaxf(fmaxf(fminf(t1x, t2x), fminf(t1y, t2y)), fminf(t1z, t2z));
+  const float tmax = fminf(fminf(fmaxf(t1x, t2x), fmaxf(t1y, t2y)), fmaxf(t1z, t2z));
+  if (!(tmax >= kVkTMin) || tmin > tmax) return false;
+  *t = tmin >= kVkTMin ? tmin : tmax;
+  return *t <= kVkTMax;
+}
+
+// The hardware keeps the least accepted t; equal t go to the lower index.
+// Accepted t are numbers, so better_hit reduces to that order.
+RD_HD void test_prim_vk(const RayPre &r, const DevPrim &p, float *bt, uint32_t *bid) {
+  const uint32_t tag = fbits(p.b[3]);
+  float t;
+  const bool h = (tag & kTagCube) ? hit_cube_vk(r, p.a, p.b, &t) : hit_sphere_vk(r, p.a, p.b, &t);
+  const uint32_t id = tag & ~kTagCube;
+  if (h && better_hit(t, id, *bt, *bid)) { *bt = t; *bid = id; }
+}
+
+// raytrace.rgen:83-89.  j.inv_view holds the inverse of look_at_rh(position,
+// target, -up) (vulkan.rs:252-272), j.inv_proj the scene's inverse projection.
+RD_HD void camera_ray_vk(const DevJob &j, uint32_t x, uint32_t y, F3 *o, F3 *d) {
+  const float u = ((float)x + 0.5f) / (float)j.width;
+  const float v = ((float)y + 0.5f) / (float)j.height;
+  const float q0 = u * 2.0f - 1.0f, q1 = v * 2.0f - 1.0f;
+  float org[3], tgt[3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    org[r] = j.inv_view[r] * 0.0f + j.inv_view[4 + r] * 0.0f + j.inv_view[8 + r] * 0.0f +
+             j.inv_view[12 + r] * 1.0f;
+    tgt[r] = j.inv_proj[r] * q0 + j.inv_proj[4 + r] * q1 + j.inv_proj[8 + r] * 1.0f +
+             j.inv_proj[12 + r] * 1.0f;
+  }
+  const F3 n = normalize(f3(tgt[0], tgt[1], tgt[2]));
+  float dir[3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+    dir[r] = j.inv_view[r] * n.x + j.inv_view[4 + r] * n.y + j.inv_view[8 + r] * n.z +
+             j.inv_view[12 + r] * 0.0f;
+  *o = f3(org[0], org[1], org[2]);
+  *d = f3(dir[0], dir[1], dir[2]);
+}
+
+// raytrace.rgen:43-48; a solid world is both colours (vulkan.rs:288-299).
+RD_HD F3 world_sample_vk(const DevJob &j, F3 d) {
+  const bool solid = j.world_kind == RD_WORLD_SOLID_COLOR;
+  const F3 top = solid ? f3(j.solid[0], j.solid[1], j.solid[2]) : f3(j.top[0], j.top[1], j.top[2]);
+  const F3 bottom =
+      solid ? f3(j.solid[0], j.solid[1], j.solid[2]) : f3(j.bottom[0], j.bottom[1], j.bottom[2]);
+  const F3 up = f3(0.0f, 1.0f, 0.0f);
+  const float cs = dot(d, up) / (sqrtf(dot(d, d)) * sqrtf(dot(up, up)));
+  const float a = (cs + 1.0f) * 0.5f;
+  return add(scale(bottom, 1.0f - a), scale(top, a));
+}
+
+// raytrace.rgen:106-172 for one hit, drawing from the pixel's PCG state in
+// the shader's order: diffuse offset (3), specular offset (3), transmission
+// test, then the Fresnel test and, when refracting, the refraction offset
+// (3); or the metallic test and, for a non-metal, the roughness test.
+RD_HD ScatterOut scatter_vk(F3 ray_d, const HitRec &h, const DevObject &ob, uint32_t &state,
+                            F3 atten, F3 light) {
+  const float rough = ob.m0[3];  // roughness * roughness (raytrace.rgen:118)
+  F3 diffuse = add(h.n, vk_rius(state));
+  if (dot(diffuse, h.n) < 0.0f) diffuse = neg(diffuse);
+  const F3 perfect = sub(ray_d, scale(h.n, 2.0f * dot(h.n, ray_d)));  // GLSL reflect
+  const F3 specular = normalize(add(perfect, scale(vk_rius(state), rough)));
+  const bool trans = vk_rand(state) < ob.m2[0];
+  F3 dir = specular;
+  if (trans) {
+    float ior = ob.m2[1];
+    if (h.front) ior = 1.0f / ior;
+    const F3 rd = normalize(ray_d);
+    const float cos_t = fminf(dot(rd, neg(h.n)), 1.0f);
+    const float q = (ior - 1.0f) / (ior + 1.0f);
+    const float r0 = q * q;
+    const float y = 1.0f - cos_t;
+    const float y2 = y * y;
+    const float refl = r0 + (1.0f - r0) * (y * (y2 * y2));
+    const float u = vk_rand(state);
+    if (refl < u && can_refract(rd, h.n, ior)) {
+      // GLSL refract(I, N, eta)
+      const float dn = dot(h.n, rd);
+      const float k = 1.0f - (ior * ior) * (1.0f - dn * dn);
+      F3 refr = f3(0.0f, 0.0f, 0.0f);
+      if (!(k < 0.0f)) refr = sub(scale(rd, ior), scale(h.n, ior * dn + sqrtf(k)));
+      dir = normalize(add(refr, scale(vk_rius(state), rough)));
+    }
+  } else if (!(vk_rand(state) < ob.m1[3])) {
+    if (vk_rand(state) < rough) dir = diffuse;
+  }
+  ScatterOut out;
+  const F3 off = trans ? dir : h.n;
+  out.o = add(h.p, scale(off, 0.001f));
+  out.d = dir;
+  if (dot(out.d, out.d) < 1e-10f) out.d = h.n;
+  out.atten = mul(atten, f3(ob.m0[0], ob.m0[1], ob.m0[2]));
+  out.light = add(light, f3(ob.m1[0], ob.m1[1], ob.m1[2]));
+  out.transmission = trans;
+  return out;
+}
+
+// imageStore of accumulated / spp into R8G8B8A8_UNORM (raytrace.rgen:186,
+// vulkan.rs:196): clamp, scale by 255, round to nearest even; NaN -> 0.
+RD_HD uint8_t tonemap_channel_vk(float acc, float spp) {
+  float v = acc / spp;
+  if (isnan(v)) return 0;
+  v = v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v);
+  return (uint8_t)rintf(v * 255.0f);
+}
+
 }  // namespace rdh

@@ -166,6 +166,7 @@ extern "C" {
     pub fn rd_get_option(r: *const rd_renderer, key: *const c_char, value: *mut i64) -> rd_status;
 
     pub fn rd_camera_update_matrices(camera: *mut rd_camera) -> rd_status;
+    pub fn rd_camera_vulkan_inverse_view(camera: *const rd_camera, out_inverse_view: *mut f32) -> rd_status;
 
     pub fn rd_scene_load_rscn(path: *const c_char, out: *mut *mut rd_scene) -> rd_status;
     pub fn rd_scene_parse_rscn(json: *const c_char, len: usize, out: *mut *mut rd_scene) -> rd_status;

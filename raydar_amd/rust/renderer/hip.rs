@@ -134,6 +134,17 @@ impl HipRenderer {
         })
     }
 
+    /// A render option of the library (rd_set_option), e.g. ("semantics", 1)
+    /// for the Vulkan backend's images; it applies from the next new_frame.
+    pub fn set_option(&mut self, key: &str, value: i64) -> Result<(), String> {
+        let k = std::ffi::CString::new(key).map_err(|e| e.to_string())?;
+        let status = unsafe { sys::rd_set_option(self.handle, k.as_ptr(), value) };
+        if status != sys::RD_OK {
+            return Err(last_error(self.handle));
+        }
+        Ok(())
+    }
+
     fn check(&self, status: sys::rd_status, call: &str) {
         if status != sys::RD_OK {
             panic!("HIP renderer: {} failed ({}): {}", call, status, last_error(self.handle));

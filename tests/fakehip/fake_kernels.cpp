@@ -424,7 +424,7 @@ hipError_t launch_accumulate(const DevJob *job, const DevTicket &t, const float4
 }
 
 hipError_t launch_tonemap(const DevJob *job, uint32_t n, float spp, uchar4 *out, DevCounters *, bool,
-                          int grid, hipStream_t st) {
+                          int grid, hipStream_t st, bool) {
   hook();
   if (grid <= 0) return hipErrorInvalidValue;
   fakehip::enqueue(st, [job, n, spp, out] {
