@@ -43,6 +43,22 @@ RD_HD F3 normalize(F3 v) { return scale(v, 1.0f / sqrtf(dot(v, v))); }
 RD_HD float signum(float v) { return isnan(v) ? v : copysignf(1.0f, v); }
 RD_HD uint32_t fbits(float v) { return __builtin_bit_cast(uint32_t, v); }
 RD_HD float bitsf(uint32_t v) { return __builtin_bit_cast(float, v); }
+// sqrtf, correctly rounded.  On the GPU, for x >= 2^-96 (no scaling needed),
+// the hardware square root s (within 1 ulp) is corrected by the sign of the
+// exact residuals x - s'*s of its neighbours s' = s -+ 1 ulp (fma is exact
+// enough to decide which of the three is the rounded root); other x take
+// sqrtf.
+RD_HD float sqrt_rn(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  if (x >= 0x1p-96f) {
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const float sd = bitsf(fbits(s) - 1u), su = bitsf(fbits(s) + 1u);
+    const float r = fmaf(-sd, s, x) <= 0.0f ? sd : s;
+    return fmaf(-su, s, x) > 0.0f ? su : r;
+  }
+#endif
+  return sqrtf(x);
+}
 
 // -------------------------------------------------------------- Philox ----
 // Philox-4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11, section 3.2).
@@ -162,7 +178,7 @@ RD_HD bool hit_sphere(const RayPre &r, const float *pa, const float *pb, float *
   const float c = r.oo - 2.0f * oC + pa[3] - pb[0];
   const float disc = k * k - r.a * c;
   if (disc < 0.0f) return false;
-  const float s = sqrtf(disc);
+  const float s = sqrt_rn(disc);
   const float t1 = (-k - s) / r.a;
   if (t1 >= 0.0f) { *t = t1; return true; }
   const float t2 = (-k + s) / r.a;
@@ -216,9 +232,18 @@ struct SlabRay {
   float ax, ay, az;     // (o + tol) * inv
   float bx, by, bz;     // (o - tol) * inv
 };
+// On the GPU the reciprocal is the hardware's (within 1 ulp instead of half
+// an ulp): the box test is a cull, and the rounding margin every box carries
+// (1e-6 L, the padding note of bvh.cpp) covers the extra 2^-24 relative error
+// of each plane distance; the near/far plane order below holds for any
+// positive scale.
 RD_HD float safe_inv(float d) {
   const float m = fabsf(d) < 1e-30f ? copysignf(1e-30f, d) : d;
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_rcpf(m);
+#else
   return 1.0f / m;
+#endif
 }
 // Whether the box walk answers exactly as the scan would: finite origin and
 // direction, and no direction component that safe_inv would clamp unless it
@@ -481,7 +506,7 @@ RD_HD bool hit_sphere_vk(const RayPre &r, const float *pa, const float *pb, floa
   const float c = r.oo - 2.0f * oC + pa[3] - pb[0];
   const float disc = k * k - r.a * c;
   if (!(disc >= 0.0f)) return false;
-  const float s = sqrtf(disc);
+  const float s = sqrt_rn(disc);
   float tt = (-k - s) / r.a;
   if (!(tt >= 0.0f)) {
     tt = (-k + s) / r.a;
