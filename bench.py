@@ -132,13 +132,14 @@ def cpu_baseline_batched(specs, pixels_per_scene=2048):
     }
 
 
-def batched_vs_plain(reps=2, gap=0.0, cpu=False):
+def batched_vs_plain(reps=2, gap=0.0, cpu=False, opts=None):
     """64 clustered scenes (16-2048 primitives, 128^2..1080p, 8/32/128 spp,
     4/8/12 bounces) in one batched wavefront vs 64 sequential plain renders
     on one reused per-scene renderer (new_frame + render_frame each); each
     side warmed by one untimed pass, then timed over `reps` passes.  `gap`:
     seconds of idle GPU before every pass, so that a kernel trace of the run
-    splits into its passes."""
+    splits into its passes.  `opts`: options set on both renderers."""
+    opts = opts or {}
     import raydar_amd as ra
     from raydar_amd import scenegen
     from raydar_amd.batch import BatchRenderer, Job
@@ -146,6 +147,8 @@ def batched_vs_plain(reps=2, gap=0.0, cpu=False):
     paths = sum(j.paths for j in specs)
     out = {"jobs": len(specs), "paths": paths}
     with BatchRenderer() as br:
+        for k, v in opts.items():
+            br.set_option(k, v)
         # one untimed pass of the whole batch (launch shapes, graphs, the
         # kept job buffers), then the mean of `reps` timed passes
         time.sleep(gap)
@@ -165,7 +168,7 @@ def batched_vs_plain(reps=2, gap=0.0, cpu=False):
                                                                     "solo_paths")},
                           "solo_jobs": br.get_option("solo_jobs")}
     solo_paths = out["batched"]["options"]["solo_paths"]
-    with ra.HipRenderer() as r:
+    with ra.HipRenderer(**opts) as r:
         runs = []
         large_s = [0.0] * (reps + 1)  # time of the jobs the batch gives the solo lane
         for k in range(reps + 1):  # the first pass warms up, as for the batch
@@ -352,7 +355,8 @@ def main():
         sys.exit(2)
 
     if args.batched_only:
-        print(json.dumps({"batched": batched_vs_plain(gap=args.phase_gap)}),
+        opts = {k: int(v) for k, v in (kv.split("=", 1) for kv in args.option)}
+        print(json.dumps({"batched": batched_vs_plain(gap=args.phase_gap, opts=opts)}),
               flush=True)
         return
 

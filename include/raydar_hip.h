@@ -222,6 +222,10 @@ rd_status rd_read_accumulator(rd_renderer *r, float *out, size_t float_capacity)
  *   "refill"      closest-hit lanes: 0 = each lane traces its rays to the end
  *                 in launch order; k in 1..64 = a wave hands new rays to its
  *                 idle lanes once k or more are idle (default 48)
+ *   "lane_chunk"  closest-hit lanes: 0 = each wave takes one contiguous range of
+ *                 the path queue; c = 64..65536 (a power of two) = the queue's
+ *                 chunks of c paths are dealt round-robin to the waves
+ *                 (default 64)
  *   "lds_bvh"     scenes whose BVH nodes + primitives take at most this many
  *                 bytes are traversed from LDS (default 14336; 0 = never)
  *   "lds_objects" scenes whose objects (64 B each) take at most this many

@@ -54,6 +54,7 @@ rd_status rd_batch_create(rd_batch **out);
 void rd_batch_destroy(rd_batch *b);
 /* Keys: "block" (64/128/256), "queue" (paths in flight), "leaf" (1..8, default 1),
  * "grid" (workgroups, 0 = auto), "refill" (0..64, as rd_set_option),
+ * "lane_chunk" (0 or 64..65536, as rd_set_option, default 64),
  * "bvh4" (1 = 4-wide trees while every resident job's walk fits the
  * kernel's stack, the default; 0 = binary trees),
  * "graph" (1 = replay a captured hipGraph),

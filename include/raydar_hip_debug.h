@@ -47,6 +47,7 @@ rd_status rd_debug_trace(const rd_scene *scene, int leaf_size, int brute, const 
  * (a 4-wide stack bound above the kernel's, a tree too large for LDS, ...). */
 typedef struct rd_extend_variant {
   int32_t bvh4, lds, wide, jobs, refill, block, grid;
+  int32_t lane_chunk; /* 0, or the queue chunk (64..65536) the waves are dealt (render option "lane_chunk") */
 } rd_extend_variant;
 rd_status rd_debug_extend(const rd_scene *scene, int leaf_size, const rd_extend_variant *variant,
                           const float *rays, uint32_t n, int32_t *ids, float *ts, float *normals,

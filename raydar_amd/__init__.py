@@ -294,7 +294,8 @@ def debug_trace(scene, rays, leaf=4, brute=False):
     return ids, ts, normals, front
 
 
-def debug_extend(scene, rays, leaf=1, bvh4=1, lds=0, wide=0, jobs=1, refill=48, block=256, grid=0):
+def debug_extend(scene, rays, leaf=1, bvh4=1, lds=0, wide=0, jobs=1, refill=48, block=256, grid=0,
+                 lane_chunk=0):
     """Closest hit through the production extend kernel (see rd_debug_extend).
     Returns (ids, ts, normals, front), or None when the renderer never launches
     that kernel form for this scene (RD_ERR_UNSUPPORTED)."""
@@ -305,7 +306,7 @@ def debug_extend(scene, rays, leaf=1, bvh4=1, lds=0, wide=0, jobs=1, refill=48, 
     normals = np.zeros((n, 3), np.float32)
     front = np.zeros(n, np.uint8)
     s = scene.as_rd()
-    v = abi.RdExtendVariant(bvh4, lds, wide, jobs, refill, block, grid)
+    v = abi.RdExtendVariant(bvh4, lds, wide, jobs, refill, block, grid, lane_chunk)
     st = lib().rd_debug_extend(C.byref(s), leaf, C.byref(v), _ptr(rays), n, _ptr(ids), _ptr(ts),
                                _ptr(normals), _ptr(front))
     if st == abi.RD_ERR_UNSUPPORTED:

@@ -99,7 +99,8 @@ class RdKernelTimes(C.Structure):
 
 
 class RdExtendVariant(C.Structure):
-    _fields_ = [(n, C.c_int32) for n in ("bvh4", "lds", "wide", "jobs", "refill", "block", "grid")]
+    _fields_ = [(n, C.c_int32) for n in ("bvh4", "lds", "wide", "jobs", "refill", "block", "grid",
+                                         "lane_chunk")]
 
 
 KERNEL_NAMES = ("generate", "extend", "shade", "accumulate", "tonemap")

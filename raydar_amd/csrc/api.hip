@@ -327,6 +327,7 @@ bool option_value(const rd_renderer *r, const std::string &k, int64_t *v) {
   else if (k == "iterations") *v = (int64_t)r->eng.iterations;
   else if (k == "num_cus") *v = r->eng.num_cus;
   else if (k == "semantics") *v = o.vk;
+  else if (k == "lane_chunk") *v = o.lane_chunk_log2 ? (1ll << o.lane_chunk_log2) : 0;
   else return false;
   return true;
 }
@@ -700,6 +701,10 @@ rd_status rd_set_option(rd_renderer *r, const char *key, int64_t v) {
     o.sync_every = (int)v;
   } else if (k == "samples_per_pass") {
     return rd_set_samples_per_pass(r, (uint32_t)std::max<int64_t>(0, std::min<int64_t>(v, 0xffffffffll)));
+  } else if (k == "lane_chunk") {
+    if (v < 0 || (v > 0 && v < 64) || v > 65536 || (v & (v - 1)) != 0)
+      return r->fail(RD_ERR_INVALID_ARGUMENT, "lane_chunk must be 0 or a power of two in 64..65536");
+    o.lane_chunk_log2 = v ? (uint32_t)__builtin_ctzll((unsigned long long)v) : 0u;
   } else if (k == "semantics") {
     if (v != 0 && v != 1) return r->fail(RD_ERR_INVALID_ARGUMENT, "semantics must be 0 (CPU renderer) or 1 (Vulkan renderer)");
     o.vk = (int)v;
@@ -845,7 +850,9 @@ rd_status rd_debug_extend(const rd_scene *scene, int leaf, const rd_extend_varia
                           uint8_t *front) {
   if (!v || !rays || !ids || !ts || !normals || !front) return RD_ERR_INVALID_ARGUMENT;
   if ((v->block != 64 && v->block != 128 && v->block != 256) || v->jobs < 1 || v->jobs > 64 ||
-      v->refill < 0 || v->refill > 64 || v->grid < 0 || v->grid > (1 << 16) || n > (1u << 26)) {
+      v->refill < 0 || v->refill > 64 || v->grid < 0 || v->grid > (1 << 16) || n > (1u << 26) ||
+      v->lane_chunk < 0 || (v->lane_chunk > 0 && v->lane_chunk < 64) || v->lane_chunk > 65536 ||
+      (v->lane_chunk & (v->lane_chunk - 1)) != 0) {
     rdh::set_global_error("rd_debug_extend: variant out of range");
     return RD_ERR_INVALID_ARGUMENT;
   }
@@ -948,6 +955,7 @@ rd_status rd_debug_extend(const rd_scene *scene, int leaf, const rd_extend_varia
   a.track_done = 0;
   a.wide_stack = v->wide ? 1 : 0;
   a.refill = v->refill;
+  a.chunk_log2 = v->lane_chunk ? (uint32_t)__builtin_ctz((unsigned)v->lane_chunk) : 0u;
   a.lds_bvh_bytes = lds_bytes;
   a.lds_obj_bytes = 0;
   a.stack_rows = rows;

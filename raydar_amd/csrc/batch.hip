@@ -452,6 +452,7 @@ class Streamer {
     graph_grid_ = eng.opt.grid;
     graph_brute_ = eng.opt.brute_force;
     graph_refill_ = eng.opt.refill;
+    graph_chunk_ = eng.opt.lane_chunk_log2;
     graph_shade_ = eng.opt.shade_blocks;
     graph_wide_ = wide_;
     graph_w4_ = a.bvh4 != 0;
@@ -459,6 +460,7 @@ class Streamer {
     return true;
   }
   bool graph_wide_ = false, graph_w4_ = false;
+  uint32_t graph_chunk_ = 0;
   uint64_t graph_gen_ = 0;
 
   bool run_iterations() {
@@ -474,6 +476,7 @@ class Streamer {
     }
     if (!graph_ || graph_block_ != eng.opt.block || graph_grid_ != eng.opt.grid ||
         graph_brute_ != eng.opt.brute_force || graph_refill_ != eng.opt.refill || graph_shade_ != eng.opt.shade_blocks ||
+        graph_chunk_ != eng.opt.lane_chunk_log2 ||
         graph_wide_ != wide_ || graph_w4_ != want_w4() ||
         graph_gen_ != eng.generation) {
       if (!capture()) return false;
@@ -1098,6 +1101,12 @@ bool engine_option(rdh::Streamer &st, const std::string &k, int64_t v, std::stri
   } else if (k == "refill") {
     if (v < 0 || v > 64) { *err = "refill must be in 0..64"; return false; }
     o.refill = (int)v;
+  } else if (k == "lane_chunk") {
+    if (v < 0 || (v > 0 && v < 64) || v > 65536 || (v & (v - 1)) != 0) {
+      *err = "lane_chunk must be 0 or a power of two in 64..65536";
+      return false;
+    }
+    o.lane_chunk_log2 = v ? (uint32_t)__builtin_ctzll((unsigned long long)v) : 0u;
   } else if (k == "shade_blocks") {
     if (v < 0 || v > 64) { *err = "shade_blocks must be in 0..64"; return false; }
     o.shade_blocks = (int)v;
@@ -1136,6 +1145,7 @@ bool engine_option_get(const rdh::Streamer &st, const std::string &k, int64_t *v
   else if (k == "leaf") *v = o.leaf;
   else if (k == "grid") *v = o.grid;
   else if (k == "refill") *v = o.refill;
+  else if (k == "lane_chunk") *v = o.lane_chunk_log2 ? (1ll << o.lane_chunk_log2) : 0;
   else if (k == "shade_blocks") *v = o.shade_blocks;
   else if (k == "bvh4") *v = o.bvh4;
   else if (k == "graph") *v = st.use_graph;

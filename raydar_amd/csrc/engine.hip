@@ -336,6 +336,7 @@ IterArgs Engine::iter_args(const DevJob *d_jobs) const {
   a.bvh4 = 0;
   a.leaf1 = 0;
   a.vk = opt.vk;
+  a.chunk_log2 = opt.lane_chunk_log2;
   a.slots = (uint32_t)kSegments * seg_cap;
   a.ticket_cap = ticket_alloc;
   a.job_cap = 1;

@@ -84,6 +84,9 @@ struct EngineOptions {
   int pipes = 0;                  // one-scene passes: independent streams the
                                   // samples are divided over (1..4; 0 = 2 for
                                   // LDS-resident scenes, else 3)
+  uint32_t lane_chunk_log2 = 6;   // closest-hit lanes: the queue's chunks of 64 paths
+                                  // dealt round-robin to the waves (0: one contiguous
+                                  // range per wave; lone extend launches 10-22% slower)
   int vk = 0;                     // 1: the Vulkan renderer's semantics (per-scene
                                   // renderer only)
 };

@@ -448,8 +448,31 @@ void k_extend_lanes(IterArgs a, int p) {
   const uint32_t n = pl.n;
   const uint32_t waves = gridDim.x * (BLOCK / 64);
   const uint32_t wave = blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6);
-  uint32_t c_next = (uint32_t)((uint64_t)n * wave / waves);
-  const uint32_t c_end = (uint32_t)((uint64_t)n * (wave + 1) / waves);
+  // A wave's share of the queue: one contiguous range (chunk_log2 = 0), or
+  // the queue's chunks of 2^chunk_log2 >= 64 consecutive indices dealt
+  // round-robin (wave w takes chunks w, w + waves, ...).  The queue keeps
+  // neighbouring pixels together, so a contiguous share can be all sky or all
+  // clutter and its wave ends early or late; dealt chunks keep neighbours
+  // together only within a chunk and give every wave a sample of the whole
+  // queue.  c_next and c_end count the wave's own indices (scalar registers:
+  // both are wave-uniform); a refill of at most 64 lanes spans at most two
+  // chunks, whose queue offsets are computed once per refill.
+  const uint32_t cl = a.chunk_log2;
+  const uint32_t cmask = (1u << cl) - 1u;
+  uint32_t c_next, c_end;
+  if (cl) {
+    const uint32_t nc = n >> cl, rem = n & cmask;
+    const uint32_t full = nc > wave ? (nc - wave - 1u) / waves + 1u : 0u;
+    c_next = 0;
+    c_end = (full << cl) + (nc % waves == wave ? rem : 0u);
+  } else {
+    c_next = (uint32_t)((uint64_t)n * wave / waves);
+    c_end = (uint32_t)((uint64_t)n * (wave + 1) / waves);
+  }
+  c_next = __builtin_amdgcn_readfirstlane(c_next);
+  c_end = __builtin_amdgcn_readfirstlane(c_end);
+  // queue index of the wave's own index c (cl > 0)
+  auto queue_index = [&](uint32_t c) { return ((((c >> cl) * waves + wave) << cl) | (c & cmask)); };
   bool more = c_next < c_end;  // wave-uniform: the wave still has rays to hand out
   StackT *stk = stack + threadIdx.x;
   const uint32_t refill = (uint32_t)a.refill;
@@ -484,8 +507,14 @@ void k_extend_lanes(IterArgs a, int p) {
       const uint32_t take = min((uint32_t)__popcll(idle), c_end - c_next);
       const uint32_t rank = lanes_below(idle);
       const bool got = !active && rank < take;
-      const uint32_t mine = c_next + rank;
-      c_next += take;
+      uint32_t mine = c_next + rank;
+      if (cl) {
+        // lanes below `left` stay in c_next's chunk, the others start the next
+        const uint32_t left = cmask + 1u - (c_next & cmask);
+        const uint32_t b0 = queue_index(c_next), b1 = queue_index(c_next + left) - left;
+        mine = (rank < left ? b0 : b1) + rank;
+      }
+      c_next = __builtin_amdgcn_readfirstlane(c_next + take);
       more = c_next < c_end;
       if (got) {
         k = chk(slot_of(pl, a.seg_cap, mine, seg), a.slots, K_SLOT, a.ctl);

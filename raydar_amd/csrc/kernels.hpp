@@ -47,6 +47,9 @@ struct IterArgs {
                                     // each shade workgroup copies into LDS
   int bvh4;                         // single job: traverse DevJob::nodes4
   int leaf1;                        // single 4-wide job whose leaves hold one primitive each
+  uint32_t chunk_log2;               // extend lanes: 0 = each wave takes one contiguous
+                                    // range of the queue; c > 0 = chunks of 2^c indices
+                                    // dealt round-robin to the waves
   int vk;                           // 1: the Vulkan renderer's semantics (single job,
                                     // block 256, one sample per pass, no counting)
   // Buffer sizes, read only by bounds-checked builds (RDH_CHECKED):

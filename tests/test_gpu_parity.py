@@ -117,6 +117,11 @@ EXTEND_VARIANTS = {
     "w4_lds_leaf4_grid2": dict(bvh4=1, lds=1, leaf=4, block=64, grid=2),
     # leaf 1 (the default above) takes the one-primitive-per-leaf kernels; leaf 2 the general ones
     "w4_global_leaf2": dict(bvh4=1, lds=0, leaf=2),
+    # the queue dealt to the waves in chunks (render option lane_chunk)
+    "w4_lds_chunk64": dict(bvh4=1, lds=1, lane_chunk=64),
+    "w4_global_chunk64_grid7": dict(bvh4=1, lds=0, lane_chunk=64, grid=7),
+    "multi_w4_chunk128": dict(bvh4=1, jobs=3, lane_chunk=128),
+    "bin_global32_chunk256": dict(bvh4=0, wide=1, lane_chunk=256),
 }
 
 

@@ -45,7 +45,9 @@ def _gpu_acc(scene, spp, bounces, seed, **opts):
                                   dict(bvh4=0, lds_bvh=0), dict(bvh4=0, leaf=8, refill=16),
                                   dict(pipes=1), dict(pipes=3, queue=65536), dict(pipes=4, queue=4096),
                                   dict(pipes=4, sync_every=1, lds_bvh=0, queue=1 << 16),
-                                  dict(lds_objects=0), dict(lds_objects=0, lds_bvh=0, pipes=2)])
+                                  dict(lds_objects=0), dict(lds_objects=0, lds_bvh=0, pipes=2),
+                                  dict(lane_chunk=0), dict(lane_chunk=64), dict(lane_chunk=64, refill=1, grid=7),
+                                  dict(lane_chunk=4096, lds_bvh=0, block=128)])
 def test_plain_small_matches_oracle(opts):
     sc = scenegen.plain_small(0)
     img, acc = _gpu_acc(sc, 16, 12, 5, **opts)
