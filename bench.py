@@ -198,6 +198,14 @@ def batched_vs_plain(reps=2, gap=0.0, cpu=False, opts=None):
                                "large_jobs_s": round(dt_large, 4),
                                "bound_if_small_jobs_were_free": round(dt_p / dt_large, 3)
                                if dt_large > 0 else None}
+    if dt_large > 0:
+        # the whole batch at the large jobs' plain rate (big passes, no
+        # per-frame overheads): if the small jobs' paths cost as much as the
+        # large jobs' (the same scene generator, spp and bounce choices), a
+        # batch at that rate beats the sequential renders by this much
+        large_rate = out["sequential_plain"]["large_job_paths"] / dt_large
+        out["sequential_plain"]["large_jobs_mpaths_per_s"] = round(large_rate / 1e6, 1)
+        out["sequential_plain"]["bound_at_large_job_rate"] = round(dt_p / (paths / large_rate), 3)
     out["batched_over_plain"] = round(dt_p / dt_b, 3)
     if cpu:
         print("# cpu baseline on the batch...", file=sys.stderr, flush=True)
