@@ -489,9 +489,13 @@ rd_status rd_render_frame_noalloc(rd_renderer *r, const rd_scene *scene, uint8_t
   const uint64_t o_prims = take(sizeof(rdh::DevPrim) * b.prims.size());
   const uint64_t o_obj = take(sizeof(rdh::DevObject) * b.objects.size());
   const uint64_t o_acc = take(16 * npix), o_img = take(4 * npix);
-  uint64_t o_q[2][14];
-  for (int p = 0; p < 2; ++p)
-    for (int a = 0; a < 14; ++a) o_q[p][a] = take(4 * slots);
+  uint64_t o_q[2][4];
+  for (int p = 0; p < 2; ++p) {
+    o_q[p][0] = take(sizeof(rdh::QRay) * slots);
+    o_q[p][1] = take(sizeof(rdh::QWord) * slots);
+    o_q[p][2] = take(sizeof(rdh::QAtt) * slots);
+    o_q[p][3] = take(sizeof(rdh::QLight) * slots);
+  }
   const uint64_t o_hit_t = take(4 * slots), o_hit_id = take(4 * slots);
   const uint64_t o_L = take(0);
   // the radiance arena takes the rest: at least one sample of every pixel
@@ -505,11 +509,10 @@ rd_status rd_render_frame_noalloc(rd_renderer *r, const rd_scene *scene, uint8_t
 
   rdh::ExternalBuffers eb{};
   for (int p = 0; p < 2; ++p) {
-    float **f[] = {&eb.q[p].ox, &eb.q[p].oy, &eb.q[p].oz, &eb.q[p].dx, &eb.q[p].dy, &eb.q[p].dz,
-                   &eb.q[p].ax, &eb.q[p].ay, &eb.q[p].az, &eb.q[p].lx, &eb.q[p].ly, &eb.q[p].lz};
-    for (int a = 0; a < 12; ++a) *f[a] = reinterpret_cast<float *>(base + o_q[p][a]);
-    eb.q[p].tb = reinterpret_cast<uint32_t *>(base + o_q[p][12]);
-    eb.q[p].pidx = reinterpret_cast<uint32_t *>(base + o_q[p][13]);
+    eb.q[p].ray = reinterpret_cast<rdh::QRay *>(base + o_q[p][0]);
+    eb.q[p].word = reinterpret_cast<rdh::QWord *>(base + o_q[p][1]);
+    eb.q[p].att = reinterpret_cast<rdh::QAtt *>(base + o_q[p][2]);
+    eb.q[p].light = reinterpret_cast<rdh::QLight *>(base + o_q[p][3]);
   }
   eb.hit_t = reinterpret_cast<float *>(base + o_hit_t);
   eb.hit_id = reinterpret_cast<uint32_t *>(base + o_hit_id);
@@ -909,13 +912,13 @@ rd_status rd_debug_extend(const rd_scene *scene, int leaf, const rd_extend_varia
   // consumer order of the iteration is the ray order and slot i holds ray i.
   const uint32_t seg_cap = std::max<uint32_t>(1, (n + rdh::kSegments - 1) / rdh::kSegments);
   const uint64_t slots = (uint64_t)rdh::kSegments * seg_cap;
-  std::vector<float> comp[6];
-  for (int a = 0; a < 6; ++a) {
-    comp[a].assign(slots, 0.0f);
-    for (uint32_t i = 0; i < n; ++i) comp[a][i] = rays[6ull * i + a];
+  std::vector<rdh::QRay> qray(slots, rdh::QRay{});
+  std::vector<rdh::QWord> qword(slots, rdh::QWord{});
+  for (uint32_t i = 0; i < n; ++i) {
+    const float *r6 = rays + 6ull * i;
+    qray[i] = rdh::QRay{r6[0], r6[1], r6[2], r6[3]};
+    qword[i] = rdh::QWord{r6[4], r6[5], (single ? 0u : i % (uint32_t)v->jobs) << 8, 0u};
   }
-  std::vector<uint32_t> tb(slots, 0), pidx(slots, 0);
-  for (uint32_t i = 0; i < n; ++i) tb[i] = (single ? 0u : i % (uint32_t)v->jobs) << 8;
   rdh::QueueCtl ctl_h;
   std::memset(&ctl_h, 0, sizeof(ctl_h));
   for (int s = 0; s < rdh::kSegments; ++s) {
@@ -925,14 +928,11 @@ rd_status rd_debug_extend(const rd_scene *scene, int leaf, const rd_extend_varia
   ctl_h.n_tickets = (uint32_t)v->jobs;
   rdh::PathQueue q[2];
   for (int p = 0; p < 2; ++p) {
-    float **f[] = {&q[p].ox, &q[p].oy, &q[p].oz, &q[p].dx, &q[p].dy, &q[p].dz,
-                   &q[p].ax, &q[p].ay, &q[p].az, &q[p].lx, &q[p].ly, &q[p].lz};
-    for (int a = 0; a < 12; ++a) *f[a] = c.dev<float>(p == 0 && a < 6 ? comp[a].data() : nullptr, slots);
-    q[p].tb = c.dev<uint32_t>(p == 0 ? tb.data() : nullptr, slots);
-    q[p].pidx = c.dev<uint32_t>(p == 0 ? pidx.data() : nullptr, slots);
-    for (int a = 0; a < 12; ++a)
-      if (!*f[a]) return RD_ERR_OUT_OF_MEMORY;
-    if (!q[p].tb || !q[p].pidx) return RD_ERR_OUT_OF_MEMORY;
+    q[p].ray = c.dev<rdh::QRay>(p == 0 ? qray.data() : nullptr, slots);
+    q[p].word = c.dev<rdh::QWord>(p == 0 ? qword.data() : nullptr, slots);
+    q[p].att = c.dev<rdh::QAtt>(nullptr, slots);
+    q[p].light = c.dev<rdh::QLight>(nullptr, slots);
+    if (!q[p].ray || !q[p].word || !q[p].att || !q[p].light) return RD_ERR_OUT_OF_MEMORY;
   }
   rdh::IterArgs a{};
   a.q[0] = q[0];

@@ -74,12 +74,10 @@ void Engine::bind_external(const ExternalBuffers &b, const Engine &owner) {
 void Engine::release_buffers() {
   if (external_) return;
   for (int p = 0; p < 2; ++p) {
-    float **f[] = {&q[p].ox, &q[p].oy, &q[p].oz, &q[p].dx, &q[p].dy, &q[p].dz,
-                   &q[p].ax, &q[p].ay, &q[p].az, &q[p].lx, &q[p].ly, &q[p].lz};
-    for (float **x : f)
-      if (*x) { (void)hipFree(*x); *x = nullptr; }
-    if (q[p].tb) { (void)hipFree(q[p].tb); q[p].tb = nullptr; }
-    if (q[p].pidx) { (void)hipFree(q[p].pidx); q[p].pidx = nullptr; }
+    void *f[] = {q[p].ray, q[p].word, q[p].att, q[p].light};
+    for (void *x : f)
+      if (x) (void)hipFree(x);
+    q[p] = PathQueue{};
   }
   if (hit_t) { (void)hipFree(hit_t); hit_t = nullptr; }
   if (hit_id) { (void)hipFree(hit_id); hit_id = nullptr; }
@@ -125,18 +123,15 @@ bool Engine::ensure_buffers(std::string *err) {
   if (q_alloc != opt.queue || need > seg_cap) {
     const uint64_t slots = (uint64_t)kSegments * need;
     for (int p = 0; p < 2; ++p) {
-      float **f[] = {&q[p].ox, &q[p].oy, &q[p].oz, &q[p].dx, &q[p].dy, &q[p].dz,
-                     &q[p].ax, &q[p].ay, &q[p].az, &q[p].lx, &q[p].ly, &q[p].lz};
-      for (float **x : f) {
-        if (*x) (void)hipFree(*x);
-        *x = nullptr;
-        if (!check(hipMalloc(x, sizeof(float) * slots), "hipMalloc(queue)", err)) return false;
-      }
-      if (q[p].tb) (void)hipFree(q[p].tb);
-      if (q[p].pidx) (void)hipFree(q[p].pidx);
-      q[p].tb = q[p].pidx = nullptr;
-      if (!check(hipMalloc(&q[p].tb, 4ull * slots), "hipMalloc(queue)", err)) return false;
-      if (!check(hipMalloc(&q[p].pidx, 4ull * slots), "hipMalloc(queue)", err)) return false;
+      void *f[] = {q[p].ray, q[p].word, q[p].att, q[p].light};
+      for (void *x : f)
+        if (x) (void)hipFree(x);
+      q[p] = PathQueue{};
+      if (!check(hipMalloc(&q[p].ray, sizeof(QRay) * slots), "hipMalloc(queue)", err) ||
+          !check(hipMalloc(&q[p].word, sizeof(QWord) * slots), "hipMalloc(queue)", err) ||
+          !check(hipMalloc(&q[p].att, sizeof(QAtt) * slots), "hipMalloc(queue)", err) ||
+          !check(hipMalloc(&q[p].light, sizeof(QLight) * slots), "hipMalloc(queue)", err))
+        return false;
     }
     if (hit_t) (void)hipFree(hit_t);
     if (hit_id) (void)hipFree(hit_id);

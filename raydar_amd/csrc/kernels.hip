@@ -160,9 +160,39 @@ __device__ __forceinline__ uint32_t find_ticket(const DevTicket *t, uint32_t lo,
 // space explicitly so they become global_load (not flat_load, which also
 // waits on the LDS counter the traversal stack uses).
 typedef float VecF4 __attribute__((ext_vector_type(4)));
+typedef float VecF2 __attribute__((ext_vector_type(2)));
 typedef int VecI4 __attribute__((ext_vector_type(4)));
+typedef unsigned int VecU4 __attribute__((ext_vector_type(4)));
 typedef __attribute__((address_space(1))) const VecF4 GlobalF4;
+typedef __attribute__((address_space(1))) const VecF2 GlobalF2;
 typedef __attribute__((address_space(1))) const VecI4 GlobalI4;
+typedef __attribute__((address_space(1))) const VecU4 GlobalU4;
+typedef __attribute__((address_space(1))) VecF4 GlobalF4Out;
+typedef __attribute__((address_space(1))) VecF2 GlobalF2Out;
+typedef __attribute__((address_space(1))) VecU4 GlobalU4Out;
+
+// A queued path's ray: origin and direction x from QRay, direction y and z
+// from the first half of QWord (and, with WORD, its ticket/bounce word).
+template <bool WORD>
+__device__ __forceinline__ void load_ray(const PathQueue &q, uint32_t k, F3 *o, F3 *d,
+                                         uint32_t *tb) {
+  const VecF4 a = *(GlobalF4 *)(q.ray + k);
+  *o = f3(a.x, a.y, a.z);
+  if (WORD) {
+    const VecU4 w = *(GlobalU4 *)(q.word + k);
+    *d = f3(a.w, bitsf(w.x), bitsf(w.y));
+    *tb = w.z;
+  } else {
+    const VecF2 w = *(GlobalF2 *)(q.word + k);
+    *d = f3(a.w, w.x, w.y);
+  }
+}
+// A path's ray and words, stored into slot k.
+__device__ __forceinline__ void store_ray(const PathQueue &q, uint32_t k, F3 o, F3 d, uint32_t tb,
+                                          uint32_t pidx) {
+  *(GlobalF4Out *)(q.ray + k) = VecF4{o.x, o.y, o.z, d.x};
+  *(GlobalU4Out *)(q.word + k) = VecU4{fbits(d.y), fbits(d.z), tb, pidx};
+}
 
 __device__ __forceinline__ void load_prim(const DevPrim *prims, uint32_t j, DevPrim *p) {
   GlobalF4 *s = (GlobalF4 *)(prims + j);
@@ -329,12 +359,9 @@ __global__ void k_generate(IterArgs a, int p) {
     } else {
       camera_ray(J, pixel % J.width, pixel / J.width, &o, &d);
     }
-    q.ox[k] = o.x; q.oy[k] = o.y; q.oz[k] = o.z;
-    q.dx[k] = d.x; q.dy[k] = d.y; q.dz[k] = d.z;
     // attenuation and light of a camera path are (1,1,1) and 0 (cpu.rs:253-254):
     // shade uses the constants at bounce 0, so they are not stored
-    q.tb[k] = ti << 8;
-    q.pidx[k] = pidx;
+    store_ray(q, k, o, d, ti << 8, pidx);
     ++made;
   }
   if (COUNT && made) atomicAdd(&a.counters->v[C_CAMERA], (unsigned long long)made);
@@ -356,9 +383,10 @@ void k_extend(IterArgs a, int p) {
   uint32_t rays = 0, seg = 0;
   for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < pl.n; i += stride) {
     const uint32_t k = slot_of(pl, a.seg_cap, i, seg);
-    const F3 o = f3(q.ox[k], q.oy[k], q.oz[k]);
-    const F3 d = f3(q.dx[k], q.dy[k], q.dz[k]);
-    const DevJob &J = a.jobs[SINGLE ? 0u : a.tickets[q.tb[k] >> 8].job];
+    F3 o, d;
+    uint32_t tbk = 0;
+    load_ray<!SINGLE>(q, k, &o, &d, &tbk);
+    const DevJob &J = a.jobs[SINGLE ? 0u : a.tickets[tbk >> 8].job];
     const RayPre r = ray_pre(o, d);
     float bt;
     uint32_t bid;
@@ -518,11 +546,12 @@ void k_extend_lanes(IterArgs a, int p) {
       more = c_next < c_end;
       if (got) {
         k = chk(slot_of(pl, a.seg_cap, mine, seg), a.slots, K_SLOT, a.ctl);
-        const F3 o = f3(q.ox[k], q.oy[k], q.oz[k]);
-        const F3 d = f3(q.dx[k], q.dy[k], q.dz[k]);
+        F3 o, d;
+        uint32_t tbk = 0;
+        load_ray<!SINGLE>(q, k, &o, &d, &tbk);
         const DevJob &J =
             a.jobs[SINGLE ? 0u
-                          : chk(a.tickets[chk(q.tb[k] >> 8, a.ticket_cap, K_TICKET, a.ctl)].job,
+                          : chk(a.tickets[chk(tbk >> 8, a.ticket_cap, K_TICKET, a.ctl)].job,
                                 a.job_cap, K_JOB, a.ctl)];
         const uint32_t n_prims = SINGLE ? j0_prims : J.n_prims;
         const uint32_t n_nodes = SINGLE ? j0_nodes : J.n_nodes;
@@ -832,19 +861,26 @@ void k_shade(IterArgs a, int p) {
     uint64_t l_slot = 0;
     if (i < pl.n) {
       const uint32_t k = chk(slot_of(pl, a.seg_cap, i, seg), a.slots, K_SLOT, a.ctl);
-      const F3 o = f3(q.ox[k], q.oy[k], q.oz[k]);
-      const F3 d = f3(q.dx[k], q.dy[k], q.dz[k]);
-      const uint32_t tb = q.tb[k];
+      const VecF4 ra = *(GlobalF4 *)(q.ray + k);
+      const VecU4 wd = *(GlobalU4 *)(q.word + k);
+      const F3 o = f3(ra.x, ra.y, ra.z);
+      const F3 d = f3(ra.w, bitsf(wd.x), bitsf(wd.y));
+      const uint32_t tb = wd.z;
       ti = chk(ticket_of<SINGLE>(tb), a.ticket_cap, K_TICKET, a.ctl);
       bounce = tb & 0xffu;
-      const uint32_t pw = q.pidx[k];
+      const uint32_t pw = wd.w;
       pidx = pw & kPathIndexMask;
       F3 at = f3(1.0f, 1.0f, 1.0f), li = f3(0.0f, 0.0f, 0.0f);
       if (bounce != 0) {
-        at = f3(q.ax[k], q.ay[k], q.az[k]);
+        const VecF4 aw = *(GlobalF4 *)(q.att + k);
+        at = f3(aw.x, aw.y, aw.z);
         // light stays exactly (+0, +0, +0) until an emitter or the sky adds
-        // to it; only lit paths carry it in the queue
-        if (pw & kLitFlag) li = f3(q.lx[k], q.ly[k], q.lz[k]);
+        // to it; only lit paths carry it in the queue (x beside the
+        // attenuation, y and z in QLight)
+        if (pw & kLitFlag) {
+          const VecF2 l2 = *(GlobalF2 *)(q.light + k);
+          li = f3(aw.w, l2.x, l2.y);
+        }
       }
       const DevTicket &T = a.tickets[ti];
       const DevJob &J = a.jobs[SINGLE ? 0u : chk(T.job, a.job_cap, K_JOB, a.ctl)];
@@ -910,15 +946,10 @@ void k_shade(IterArgs a, int p) {
     if (alive && block_base != 0xffffffffu) {
       const uint32_t k = chk(out_seg * a.seg_cap + block_base + wave_off[wid] + lanes_below(am), a.slots,
                              K_SLOT, a.ctl);
-      nq.ox[k] = no.x; nq.oy[k] = no.y; nq.oz[k] = no.z;
-      nq.dx[k] = nd.x; nq.dy[k] = nd.y; nq.dz[k] = nd.z;
-      nq.ax[k] = na.x; nq.ay[k] = na.y; nq.az[k] = na.z;
       const bool lit = (fbits(nl.x) | fbits(nl.y) | fbits(nl.z)) != 0u;
-      if (lit) {
-        nq.lx[k] = nl.x; nq.ly[k] = nl.y; nq.lz[k] = nl.z;
-      }
-      nq.tb[k] = (ti << 8) | bounce;
-      nq.pidx[k] = pidx | (lit ? kLitFlag : 0u);
+      store_ray(nq, k, no, nd, (ti << 8) | bounce, pidx | (lit ? kLitFlag : 0u));
+      *(GlobalF4Out *)(nq.att + k) = VecF4{na.x, na.y, na.z, nl.x};
+      if (lit) *(GlobalF2Out *)(nq.light + k) = VecF2{nl.y, nl.z};
     }
     if (finished) a.L[l_slot] = make_float4(nl.x, nl.y, nl.z, 1.0f);
     if (a.track_done) {

@@ -11,7 +11,7 @@
 //                    accumulator, RNG seed, bounce limit
 //   DevTicket        one pass of one job: `paths` consecutive path indices
 //                    s*W*H + p, s in [0, spp); radiance slots in the L arena
-//   Path queue       structure of arrays, 56 B per path in flight
+//   Path queue       four arrays of packed path fields, 56 B per path in flight
 #pragma once
 
 #include <stdint.h>
@@ -32,7 +32,7 @@ constexpr uint32_t kNoHit = 0x7fffffffu; // best-id sentinel: "no hit yet"
 // Path queue word `pidx`: the path's index in its ticket in bits 0..30 (a
 // pass holds at most 2^31 paths: l_capacity <= 2^31), and in bit 31 whether
 // the path's light is not (+0, +0, +0); the light words of a queue slot are
-// written and read only when it is set.
+// read only when it is set (y and z are also written only then).
 constexpr uint32_t kLitFlag = 0x80000000u;
 constexpr uint32_t kPathIndexMask = 0x7fffffffu;
 constexpr int kDrawWords = 8;            // random words per bounce (two Philox blocks)
@@ -118,14 +118,30 @@ struct DevTicket {
   uint32_t paths;           // spp * W * H
 };
 
-// Structure-of-arrays path queue (one of two ping-pong buffers).
+// Path queue (one of two ping-pong buffers): four arrays indexed by slot,
+// each holding a path's fields in the order the kernels use them, so that a
+// path moves in 16-byte loads and stores (a wave's accesses to consecutive
+// slots stay contiguous) and a kernel needs four base addresses per queue.
+struct alignas(16) QRay {
+  float ox, oy, oz, dx;     // origin, direction x
+};
+struct alignas(16) QWord {
+  float dy, dz;             // direction y, z
+  uint32_t tb;              // ticket << 8 | bounce
+  uint32_t pidx;            // path index inside its ticket (s_local * W*H + pixel) | lit flag
+};
+struct alignas(16) QAtt {
+  float ax, ay, az;         // attenuation (cpu.rs:254)
+  float lx;                 // light x (cpu.rs:253); valid only with the lit flag
+};
+struct alignas(8) QLight {
+  float ly, lz;             // light y, z; read and written only with the lit flag
+};
 struct PathQueue {
-  float *ox, *oy, *oz;
-  float *dx, *dy, *dz;
-  float *ax, *ay, *az;      // attenuation (cpu.rs:254)
-  float *lx, *ly, *lz;      // light (cpu.rs:253)
-  uint32_t *tb;             // ticket << 8 | bounce
-  uint32_t *pidx;           // path index inside its ticket: s_local * W*H + pixel
+  QRay *ray;
+  QWord *word;
+  QAtt *att;
+  QLight *light;
 };
 
 // Per-iteration control block in device memory (ping-pong by parity).

@@ -49,7 +49,7 @@ struct LRange {
 typedef std::tuple<const void *, uint32_t, uint64_t, uint32_t, uint32_t> TicketKey;
 
 std::mutex g_k;  // fake-kernel state
-std::map<const void *, std::vector<Run>> g_queue;      // by PathQueue::tb
+std::map<const void *, std::vector<Run>> g_queue;      // by PathQueue::word
 std::map<TicketKey, uint64_t> g_finished;             // paths finished per issued ticket
 std::map<const void *, std::vector<LRange>> g_lmap;   // radiance ranges in use, by arena
 std::map<const void *, uint64_t> g_acc;               // samples accumulated, by accumulator
@@ -127,8 +127,10 @@ bool check_args(const IterArgs &a, const LaunchShape &sh, const char *k) {
   }
   for (int p = 0; p < 2; ++p) {
     const PathQueue &q = a.q[p];
-    const void *arr[] = {q.ox, q.oy, q.oz, q.dx, q.dy, q.dz, q.ax, q.ay, q.az, q.lx, q.ly, q.lz, q.tb, q.pidx};
-    for (const void *x : arr) ok = need(x, 4ull * slots, k, "queue array") && ok;
+    ok = need(q.ray, sizeof(QRay) * slots, k, "queue array") && ok;
+    ok = need(q.word, sizeof(QWord) * slots, k, "queue array") && ok;
+    ok = need(q.att, sizeof(QAtt) * slots, k, "queue array") && ok;
+    ok = need(q.light, sizeof(QLight) * slots, k, "queue array") && ok;
   }
   ok = need(a.hit_t, 4ull * slots, k, "hit distances") && ok;
   ok = need(a.hit_id, 4ull * slots, k, "hit ids") && ok;
@@ -205,7 +207,7 @@ void fake_generate(const IterArgs a, int p, const LaunchShape sh) {
   std::lock_guard<std::mutex> g(g_k);
   if (!check_args(a, sh, "generate")) return;
   const Plan pl = plan(a, p);
-  std::vector<Run> &runs = g_queue[queue_of(a, p).tb];
+  std::vector<Run> &runs = g_queue[queue_of(a, p).word];
   uint64_t inflight = 0;
   for (const Run &r : runs) inflight += r.count;
   if (inflight != pl.n_old)
@@ -243,7 +245,7 @@ void fake_extend(const IterArgs a, int p, const LaunchShape sh) {
   std::lock_guard<std::mutex> g(g_k);
   if (!check_args(a, sh, "extend")) return;
   const Plan pl = plan(a, p);
-  const std::vector<Run> &runs = g_queue[queue_of(a, p).tb];
+  const std::vector<Run> &runs = g_queue[queue_of(a, p).word];
   uint64_t inflight = 0;
   std::set<uint32_t> seen;
   for (const Run &r : runs) {
@@ -254,7 +256,7 @@ void fake_extend(const IterArgs a, int p, const LaunchShape sh) {
     err("extend", "iteration covers " + std::to_string(pl.n_old + pl.n_new) + " paths, " +
                       std::to_string(inflight) + " are queued");
   // queue[1-p] is refilled by shade; its previous contents were consumed
-  g_queue[queue_of(a, 1 - p).tb].clear();
+  g_queue[queue_of(a, 1 - p).word].clear();
   for (int s = 0; s < kSegments; ++s) a.ctl->count[1 - p][s * kSegStride] = 0;
   a.ctl->cursor[1 - p] = pl.cursor + pl.n_new;
 }
@@ -302,8 +304,8 @@ void fake_shade(const IterArgs a, int p, const LaunchShape sh) {
        a.lds_obj_bytes > 65536))
     err("shade", "LDS object copy of " + std::to_string(a.lds_obj_bytes) + " bytes does not match the job");
   const Plan pl = plan(a, p);
-  std::vector<Run> &runs = g_queue[queue_of(a, p).tb];
-  std::vector<Run> &next = g_queue[queue_of(a, 1 - p).tb];
+  std::vector<Run> &runs = g_queue[queue_of(a, p).word];
+  std::vector<Run> &next = g_queue[queue_of(a, 1 - p).word];
   uint64_t inflight = 0, surv_total = 0;
   std::set<uint32_t> seen;
   for (const Run &r : runs) {
