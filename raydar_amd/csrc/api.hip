@@ -489,12 +489,13 @@ rd_status rd_render_frame_noalloc(rd_renderer *r, const rd_scene *scene, uint8_t
   const uint64_t o_prims = take(sizeof(rdh::DevPrim) * b.prims.size());
   const uint64_t o_obj = take(sizeof(rdh::DevObject) * b.objects.size());
   const uint64_t o_acc = take(16 * npix), o_img = take(4 * npix);
-  uint64_t o_q[2][4];
+  uint64_t o_q[2][5];
   for (int p = 0; p < 2; ++p) {
     o_q[p][0] = take(sizeof(rdh::QRay) * slots);
-    o_q[p][1] = take(sizeof(rdh::QWord) * slots);
-    o_q[p][2] = take(sizeof(rdh::QAtt) * slots);
-    o_q[p][3] = take(sizeof(rdh::QLight) * slots);
+    o_q[p][1] = take(sizeof(rdh::QDir) * slots);
+    o_q[p][2] = take(sizeof(rdh::QWord) * slots);
+    o_q[p][3] = take(sizeof(rdh::QAtt) * slots);
+    o_q[p][4] = take(sizeof(rdh::QLight) * slots);
   }
   const uint64_t o_hit_t = take(4 * slots), o_hit_id = take(4 * slots);
   const uint64_t o_L = take(0);
@@ -510,9 +511,10 @@ rd_status rd_render_frame_noalloc(rd_renderer *r, const rd_scene *scene, uint8_t
   rdh::ExternalBuffers eb{};
   for (int p = 0; p < 2; ++p) {
     eb.q[p].ray = reinterpret_cast<rdh::QRay *>(base + o_q[p][0]);
-    eb.q[p].word = reinterpret_cast<rdh::QWord *>(base + o_q[p][1]);
-    eb.q[p].att = reinterpret_cast<rdh::QAtt *>(base + o_q[p][2]);
-    eb.q[p].light = reinterpret_cast<rdh::QLight *>(base + o_q[p][3]);
+    eb.q[p].dir = reinterpret_cast<rdh::QDir *>(base + o_q[p][1]);
+    eb.q[p].word = reinterpret_cast<rdh::QWord *>(base + o_q[p][2]);
+    eb.q[p].att = reinterpret_cast<rdh::QAtt *>(base + o_q[p][3]);
+    eb.q[p].light = reinterpret_cast<rdh::QLight *>(base + o_q[p][4]);
   }
   eb.hit_t = reinterpret_cast<float *>(base + o_hit_t);
   eb.hit_id = reinterpret_cast<uint32_t *>(base + o_hit_id);
@@ -913,11 +915,13 @@ rd_status rd_debug_extend(const rd_scene *scene, int leaf, const rd_extend_varia
   const uint32_t seg_cap = std::max<uint32_t>(1, (n + rdh::kSegments - 1) / rdh::kSegments);
   const uint64_t slots = (uint64_t)rdh::kSegments * seg_cap;
   std::vector<rdh::QRay> qray(slots, rdh::QRay{});
+  std::vector<rdh::QDir> qdir(slots, rdh::QDir{});
   std::vector<rdh::QWord> qword(slots, rdh::QWord{});
   for (uint32_t i = 0; i < n; ++i) {
     const float *r6 = rays + 6ull * i;
     qray[i] = rdh::QRay{r6[0], r6[1], r6[2], r6[3]};
-    qword[i] = rdh::QWord{r6[4], r6[5], (single ? 0u : i % (uint32_t)v->jobs) << 8, 0u};
+    qdir[i] = rdh::QDir{r6[4], r6[5]};
+    qword[i] = rdh::QWord{(single ? 0u : i % (uint32_t)v->jobs) << 8, 0u};
   }
   rdh::QueueCtl ctl_h;
   std::memset(&ctl_h, 0, sizeof(ctl_h));
@@ -929,10 +933,11 @@ rd_status rd_debug_extend(const rd_scene *scene, int leaf, const rd_extend_varia
   rdh::PathQueue q[2];
   for (int p = 0; p < 2; ++p) {
     q[p].ray = c.dev<rdh::QRay>(p == 0 ? qray.data() : nullptr, slots);
+    q[p].dir = c.dev<rdh::QDir>(p == 0 ? qdir.data() : nullptr, slots);
     q[p].word = c.dev<rdh::QWord>(p == 0 ? qword.data() : nullptr, slots);
     q[p].att = c.dev<rdh::QAtt>(nullptr, slots);
     q[p].light = c.dev<rdh::QLight>(nullptr, slots);
-    if (!q[p].ray || !q[p].word || !q[p].att || !q[p].light) return RD_ERR_OUT_OF_MEMORY;
+    if (!q[p].ray || !q[p].dir || !q[p].word || !q[p].att || !q[p].light) return RD_ERR_OUT_OF_MEMORY;
   }
   rdh::IterArgs a{};
   a.q[0] = q[0];

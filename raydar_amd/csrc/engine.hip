@@ -74,7 +74,7 @@ void Engine::bind_external(const ExternalBuffers &b, const Engine &owner) {
 void Engine::release_buffers() {
   if (external_) return;
   for (int p = 0; p < 2; ++p) {
-    void *f[] = {q[p].ray, q[p].word, q[p].att, q[p].light};
+    void *f[] = {q[p].ray, q[p].dir, q[p].word, q[p].att, q[p].light};
     for (void *x : f)
       if (x) (void)hipFree(x);
     q[p] = PathQueue{};
@@ -123,11 +123,12 @@ bool Engine::ensure_buffers(std::string *err) {
   if (q_alloc != opt.queue || need > seg_cap) {
     const uint64_t slots = (uint64_t)kSegments * need;
     for (int p = 0; p < 2; ++p) {
-      void *f[] = {q[p].ray, q[p].word, q[p].att, q[p].light};
+      void *f[] = {q[p].ray, q[p].dir, q[p].word, q[p].att, q[p].light};
       for (void *x : f)
         if (x) (void)hipFree(x);
       q[p] = PathQueue{};
       if (!check(hipMalloc(&q[p].ray, sizeof(QRay) * slots), "hipMalloc(queue)", err) ||
+          !check(hipMalloc(&q[p].dir, sizeof(QDir) * slots), "hipMalloc(queue)", err) ||
           !check(hipMalloc(&q[p].word, sizeof(QWord) * slots), "hipMalloc(queue)", err) ||
           !check(hipMalloc(&q[p].att, sizeof(QAtt) * slots), "hipMalloc(queue)", err) ||
           !check(hipMalloc(&q[p].light, sizeof(QLight) * slots), "hipMalloc(queue)", err))

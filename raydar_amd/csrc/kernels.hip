@@ -163,35 +163,33 @@ typedef float VecF4 __attribute__((ext_vector_type(4)));
 typedef float VecF2 __attribute__((ext_vector_type(2)));
 typedef int VecI4 __attribute__((ext_vector_type(4)));
 typedef unsigned int VecU4 __attribute__((ext_vector_type(4)));
+typedef unsigned int VecU2 __attribute__((ext_vector_type(2)));
 typedef __attribute__((address_space(1))) const VecF4 GlobalF4;
 typedef __attribute__((address_space(1))) const VecF2 GlobalF2;
 typedef __attribute__((address_space(1))) const VecI4 GlobalI4;
 typedef __attribute__((address_space(1))) const VecU4 GlobalU4;
+typedef __attribute__((address_space(1))) const VecU2 GlobalU2;
 typedef __attribute__((address_space(1))) VecF4 GlobalF4Out;
 typedef __attribute__((address_space(1))) VecF2 GlobalF2Out;
 typedef __attribute__((address_space(1))) VecU4 GlobalU4Out;
+typedef __attribute__((address_space(1))) VecU2 GlobalU2Out;
 
-// A queued path's ray: origin and direction x from QRay, direction y and z
-// from the first half of QWord (and, with WORD, its ticket/bounce word).
+// A queued path's ray (QRay, QDir), and with WORD its ticket/bounce word.
 template <bool WORD>
 __device__ __forceinline__ void load_ray(const PathQueue &q, uint32_t k, F3 *o, F3 *d,
                                          uint32_t *tb) {
   const VecF4 a = *(GlobalF4 *)(q.ray + k);
+  const VecF2 b = *(GlobalF2 *)(q.dir + k);
   *o = f3(a.x, a.y, a.z);
-  if (WORD) {
-    const VecU4 w = *(GlobalU4 *)(q.word + k);
-    *d = f3(a.w, bitsf(w.x), bitsf(w.y));
-    *tb = w.z;
-  } else {
-    const VecF2 w = *(GlobalF2 *)(q.word + k);
-    *d = f3(a.w, w.x, w.y);
-  }
+  *d = f3(a.w, b.x, b.y);
+  if (WORD) *tb = *(__attribute__((address_space(1))) const uint32_t *)&q.word[k].tb;
 }
 // A path's ray and words, stored into slot k.
 __device__ __forceinline__ void store_ray(const PathQueue &q, uint32_t k, F3 o, F3 d, uint32_t tb,
                                           uint32_t pidx) {
   *(GlobalF4Out *)(q.ray + k) = VecF4{o.x, o.y, o.z, d.x};
-  *(GlobalU4Out *)(q.word + k) = VecU4{fbits(d.y), fbits(d.z), tb, pidx};
+  *(GlobalF2Out *)(q.dir + k) = VecF2{d.y, d.z};
+  *(GlobalU2Out *)(q.word + k) = VecU2{tb, pidx};
 }
 
 __device__ __forceinline__ void load_prim(const DevPrim *prims, uint32_t j, DevPrim *p) {
@@ -862,13 +860,14 @@ void k_shade(IterArgs a, int p) {
     if (i < pl.n) {
       const uint32_t k = chk(slot_of(pl, a.seg_cap, i, seg), a.slots, K_SLOT, a.ctl);
       const VecF4 ra = *(GlobalF4 *)(q.ray + k);
-      const VecU4 wd = *(GlobalU4 *)(q.word + k);
+      const VecF2 rd = *(GlobalF2 *)(q.dir + k);
+      const VecU2 wd = *(GlobalU2 *)(q.word + k);
       const F3 o = f3(ra.x, ra.y, ra.z);
-      const F3 d = f3(ra.w, bitsf(wd.x), bitsf(wd.y));
-      const uint32_t tb = wd.z;
+      const F3 d = f3(ra.w, rd.x, rd.y);
+      const uint32_t tb = wd.x;
       ti = chk(ticket_of<SINGLE>(tb), a.ticket_cap, K_TICKET, a.ctl);
       bounce = tb & 0xffu;
-      const uint32_t pw = wd.w;
+      const uint32_t pw = wd.y;
       pidx = pw & kPathIndexMask;
       F3 at = f3(1.0f, 1.0f, 1.0f), li = f3(0.0f, 0.0f, 0.0f);
       if (bounce != 0) {

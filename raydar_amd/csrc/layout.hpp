@@ -11,7 +11,7 @@
 //                    accumulator, RNG seed, bounce limit
 //   DevTicket        one pass of one job: `paths` consecutive path indices
 //                    s*W*H + p, s in [0, spp); radiance slots in the L arena
-//   Path queue       four arrays of packed path fields, 56 B per path in flight
+//   Path queue       five arrays of packed path fields, 56 B per path in flight
 #pragma once
 
 #include <stdint.h>
@@ -118,15 +118,18 @@ struct DevTicket {
   uint32_t paths;           // spp * W * H
 };
 
-// Path queue (one of two ping-pong buffers): four arrays indexed by slot,
+// Path queue (one of two ping-pong buffers): five arrays indexed by slot,
 // each holding a path's fields in the order the kernels use them, so that a
-// path moves in 16-byte loads and stores (a wave's accesses to consecutive
-// slots stay contiguous) and a kernel needs four base addresses per queue.
+// path moves in 16- and 8-byte loads and stores (a wave's accesses to
+// consecutive slots stay contiguous), the closest-hit kernel reads exactly
+// its ray (QRay + QDir), and a kernel needs five base addresses per queue.
 struct alignas(16) QRay {
   float ox, oy, oz, dx;     // origin, direction x
 };
-struct alignas(16) QWord {
+struct alignas(8) QDir {
   float dy, dz;             // direction y, z
+};
+struct alignas(8) QWord {
   uint32_t tb;              // ticket << 8 | bounce
   uint32_t pidx;            // path index inside its ticket (s_local * W*H + pixel) | lit flag
 };
@@ -139,6 +142,7 @@ struct alignas(8) QLight {
 };
 struct PathQueue {
   QRay *ray;
+  QDir *dir;
   QWord *word;
   QAtt *att;
   QLight *light;

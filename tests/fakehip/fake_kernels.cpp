@@ -128,6 +128,7 @@ bool check_args(const IterArgs &a, const LaunchShape &sh, const char *k) {
   for (int p = 0; p < 2; ++p) {
     const PathQueue &q = a.q[p];
     ok = need(q.ray, sizeof(QRay) * slots, k, "queue array") && ok;
+    ok = need(q.dir, sizeof(QDir) * slots, k, "queue array") && ok;
     ok = need(q.word, sizeof(QWord) * slots, k, "queue array") && ok;
     ok = need(q.att, sizeof(QAtt) * slots, k, "queue array") && ok;
     ok = need(q.light, sizeof(QLight) * slots, k, "queue array") && ok;
