@@ -169,6 +169,19 @@ typedef __attribute__((address_space(1))) const VecF2 GlobalF2;
 typedef __attribute__((address_space(1))) const VecI4 GlobalI4;
 typedef __attribute__((address_space(1))) const VecU4 GlobalU4;
 typedef __attribute__((address_space(1))) const VecU2 GlobalU2;
+// A record of a table that no kernel writes during a launch, read through
+// the constant address space: at a wave-uniform address that is a scalar
+// load.  The pointer is rebuilt from its integer value, which the compiler
+// cannot fold back to a global pointer (a plain cast it folds, and the loads
+// stay vector loads).  The host pass of the compiler only checks the body.
+template <class T>
+__device__ __forceinline__ T load_constant(const T *p) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return *(const __attribute__((address_space(4))) T *)(uintptr_t)p;
+#else
+  return *p;
+#endif
+}
 typedef __attribute__((address_space(1))) VecF4 GlobalF4Out;
 typedef __attribute__((address_space(1))) VecF2 GlobalF2Out;
 typedef __attribute__((address_space(1))) VecU4 GlobalU4Out;
@@ -813,9 +826,9 @@ void k_extend_lanes(IterArgs a, int p) {
 // in its own segment (blockIdx % kSegments) of the next queue.  The host
 // sizes segments so that the blocks sharing one can never overfill it; the
 // overflow flag is a guard, not a mode of operation.
-// Single-job shade fits 72 VGPRs (8 B of scratch) for 7 waves/SIMD instead
-// of 6 at 75: plain_1080p +1.1%, clustered_1080p +2.1%.  The multi-job
-// variant (94 VGPRs) would spill 64 B, so it keeps the compiler's choice.
+// Single-job shade is held to at least 7 waves/SIMD (it needs 63 VGPRs and
+// runs at 8).  The multi-job variant (92 VGPRs, 5 waves) would spill at 6,
+// so it keeps the compiler's choice.
 constexpr int shade_min_waves(bool single) { return single ? 7 : 1; }
 
 // VK (Vulkan semantics, single job, one sample per pass): the scatter and
@@ -838,6 +851,17 @@ void k_shade(IterArgs a, int p) {
   // constant): the object read follows the hit id, and from LDS it does not
   // add a second global-memory latency to every path.
   extern __shared__ __align__(16) unsigned char dynamic_lds[];
+  // One job (SINGLE): the job and its one ticket are read here, before the
+  // kernel's first global store, as scalar loads whose values stay in scalar
+  // registers.  Read through the pointers inside the loop, after stores,
+  // each field would be a vector load and a memory round trip in every
+  // path's chain.
+  DevJob job0;
+  DevTicket ticket0;
+  if (SINGLE) {
+    job0 = a.jobs[0];
+    ticket0 = a.tickets[0];
+  }
   const bool lds_obj = SINGLE && a.lds_obj_bytes != 0;
   VecF4 *lds_objects = reinterpret_cast<VecF4 *>(dynamic_lds);
   if (lds_obj) {
@@ -853,23 +877,32 @@ void k_shade(IterArgs a, int p) {
   uint32_t c_sph = 0, c_cube = 0, c_glass = 0, c_miss = 0, seg = 0;
   for (uint32_t base = blockIdx.x * BLOCK; base < pl.n; base += gridDim.x * BLOCK) {
     const uint32_t i = base + threadIdx.x;
+    const bool has = i < pl.n;
     bool alive = false, finished = false;
-    uint32_t ti = 0, pidx = 0, bounce = 0;
-    F3 no = f3(0, 0, 0), nd = no, na = no, nl = no;
+    uint32_t ti = 0, pidx = 0, bounce = 0, id = kNoHit;
+    float hit_t = 0.0f;
+    F3 o = f3(0, 0, 0), d = o, at = o, li = o;
+    F3 no = o, nd = o, na = o, nl = o;
     uint64_t l_slot = 0;
-    if (i < pl.n) {
+    if (has) {
       const uint32_t k = chk(slot_of(pl, a.seg_cap, i, seg), a.slots, K_SLOT, a.ctl);
+      // the ray, its words and its hit in one batch of reads; attenuation
+      // and light only where the bounce and lit flag say they exist (read
+      // whatever the flags say, they cost more in traffic than the round
+      // trip they save)
       const VecF4 ra = *(GlobalF4 *)(q.ray + k);
       const VecF2 rd = *(GlobalF2 *)(q.dir + k);
       const VecU2 wd = *(GlobalU2 *)(q.word + k);
-      const F3 o = f3(ra.x, ra.y, ra.z);
-      const F3 d = f3(ra.w, rd.x, rd.y);
+      id = a.hit_id[k];
+      hit_t = a.hit_t[k];
+      o = f3(ra.x, ra.y, ra.z);
+      d = f3(ra.w, rd.x, rd.y);
       const uint32_t tb = wd.x;
       ti = chk(ticket_of<SINGLE>(tb), a.ticket_cap, K_TICKET, a.ctl);
       bounce = tb & 0xffu;
       const uint32_t pw = wd.y;
       pidx = pw & kPathIndexMask;
-      F3 at = f3(1.0f, 1.0f, 1.0f), li = f3(0.0f, 0.0f, 0.0f);
+      at = f3(1.0f, 1.0f, 1.0f);
       if (bounce != 0) {
         const VecF4 aw = *(GlobalF4 *)(q.att + k);
         at = f3(aw.x, aw.y, aw.z);
@@ -881,16 +914,16 @@ void k_shade(IterArgs a, int p) {
           li = f3(aw.w, l2.x, l2.y);
         }
       }
-      const DevTicket &T = a.tickets[ti];
-      const DevJob &J = a.jobs[SINGLE ? 0u : chk(T.job, a.job_cap, K_JOB, a.ctl)];
+    }
+    // the path's bounce with its ticket T and job J
+    auto shade_path = [&](const DevJob &J, const DevTicket &T) {
       l_slot = chk((unsigned long long)(T.l_offset + pidx), a.l_cap, K_RADIANCE, a.ctl);
-      const uint32_t id = a.hit_id[k];
       if (id != kNoHit) {
         const uint32_t oid = chk(id, J.n_prims, K_OBJECT, a.ctl);
         const DevObject ob = lds_obj ? object_from(lds_objects[4 * oid], lds_objects[4 * oid + 1],
                                                    lds_objects[4 * oid + 2], lds_objects[4 * oid + 3])
                                      : load_object(J.objects, oid);
-        const HitRec h = closest_hit(o, d, a.hit_t[k], ob);
+        const HitRec h = closest_hit(o, d, hit_t, ob);
         const uint32_t npix = J.width * J.height;
         ScatterOut so;
         if (VK) {
@@ -921,6 +954,30 @@ void k_shade(IterArgs a, int p) {
         nl = add(li, mul(VK ? world_sample_vk(J, d) : world_sample(J, d), at));
         finished = true;
         if (COUNT) c_miss++;
+      }
+    };
+    if (SINGLE) {
+      if (has) shade_path(job0, ticket0);
+    } else {
+      // A wave's paths nearly always share one ticket (the queue keeps a
+      // ticket's paths together): one round per distinct ticket, with the
+      // ticket and its job read at a wave-uniform index from the constant
+      // address space (neither changes during a launch), i.e. as scalar
+      // loads that the scalar cache serves, instead of two dependent vector
+      // loads and their job-field loads in every path's chain.
+      uint64_t todo = __ballot(has);
+      while (todo) {
+        const int leader = __ffsll((unsigned long long)todo) - 1;
+        const uint32_t t = (uint32_t)__builtin_amdgcn_readlane((int)ti, leader);
+        // membership is taken from the ballot's bit, not from ti == t: under
+        // that condition the compiler would read the tables at ti again
+        const uint64_t same = __ballot(has && ti == t);
+        todo &= ~same;
+        if ((same >> lane) & 1ull) {
+          const DevTicket T = load_constant(a.tickets + t);
+          const DevJob J = load_constant(a.jobs + chk(T.job, a.job_cap, K_JOB, a.ctl));
+          shade_path(J, T);
+        }
       }
     }
     const uint64_t am = __ballot(alive);
