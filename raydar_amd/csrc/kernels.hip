@@ -555,21 +555,8 @@ void k_extend_lanes(IterArgs a, int p) {
       }
       c_next = __builtin_amdgcn_readfirstlane(c_next + take);
       more = c_next < c_end;
-      if (got) {
-        k = chk(slot_of(pl, a.seg_cap, mine, seg), a.slots, K_SLOT, a.ctl);
-        F3 o, d;
-        uint32_t tbk = 0;
-        load_ray<!SINGLE>(q, k, &o, &d, &tbk);
-        const DevJob &J =
-            a.jobs[SINGLE ? 0u
-                          : chk(a.tickets[chk(tbk >> 8, a.ticket_cap, K_TICKET, a.ctl)].job,
-                                a.job_cap, K_JOB, a.ctl)];
-        const uint32_t n_prims = SINGLE ? j0_prims : J.n_prims;
-        const uint32_t n_nodes = SINGLE ? j0_nodes : J.n_nodes;
-#ifdef RDH_CHECKED
-        cur_nodes = W4 ? (SINGLE ? j0_nodes4 : J.n_nodes4) : n_nodes;
-        cur_prims = n_prims;
-#endif
+      // a refilled lane's start: the exact linear scan, or a walk from the root
+      auto start = [&](F3 o, F3 d, uint32_t n_prims, uint32_t n_nodes, float r0) {
         r = ray_pre(o, d);
         bt = 0.0f;
         bid = kNoHit;
@@ -578,7 +565,7 @@ void k_extend_lanes(IterArgs a, int p) {
           // answer for rays the box walk cannot represent (walkable()); done at once
           for (uint32_t j = 0; j < n_prims; ++j) {
             DevPrim pr;
-            load_prim(SINGLE ? prims : J.prims, j, &pr);
+            load_prim(prims, j, &pr);
             if (COUNT) {
               if (fbits(pr.b[3]) & kTagCube) tc.cube++;
               else tc.sphere++;
@@ -590,18 +577,90 @@ void k_extend_lanes(IterArgs a, int p) {
           a.hit_id[k] = bid;
           ++rays;
         } else {
-          if (!SINGLE) {
-            nodes = J.nodes;
-            nodes4 = J.nodes4;
-            prims = J.prims;
-          }
-          s = slab_ray(r, ray_tolerance(r, SINGLE ? j0_r0 : J.r0_tol));
+          s = slab_ray(r, ray_tolerance(r, r0));
           if (W4) nf = slab_nf(s);
           node = 0;
           sp = 0;
           tcull = cull0;  // cull_limit of "no hit yet"
           active = true;
         }
+      };
+      if (SINGLE) {
+        // (the start written out again: through the lambda the compiler
+        // allocates this kernel's registers differently)
+        if (got) {
+          k = chk(slot_of(pl, a.seg_cap, mine, seg), a.slots, K_SLOT, a.ctl);
+          F3 o, d;
+          uint32_t tbk = 0;
+          load_ray<false>(q, k, &o, &d, &tbk);
+#ifdef RDH_CHECKED
+          cur_nodes = W4 ? j0_nodes4 : j0_nodes;
+          cur_prims = j0_prims;
+#endif
+          r = ray_pre(o, d);
+          bt = 0.0f;
+          bid = kNoHit;
+          if (j0_prims == 0 || a.brute_force || !walkable(r) || j0_nodes == 0) {
+            for (uint32_t j = 0; j < j0_prims; ++j) {
+              DevPrim pr;
+              load_prim(prims, j, &pr);
+              if (COUNT) {
+                if (fbits(pr.b[3]) & kTagCube) tc.cube++;
+                else tc.sphere++;
+              }
+              if (VK) test_prim_vk(r, pr, &bt, &bid);
+              else test_prim(r, pr, &bt, &bid);
+            }
+            a.hit_t[k] = bid == kNoHit ? 0.0f : bt;
+            a.hit_id[k] = bid;
+            ++rays;
+          } else {
+            s = slab_ray(r, ray_tolerance(r, j0_r0));
+            if (W4) nf = slab_nf(s);
+            node = 0;
+            sp = 0;
+            tcull = cull0;
+            active = true;
+          }
+        }
+      } else {
+        // several jobs: one round per distinct ticket among the refilled
+        // lanes (nearly always one), reading the ticket and job records at
+        // a wave-uniform index as scalar loads; a refilled lane was idle, so
+        // its tree pointers are free to take its new job's
+        F3 o = f3(0.0f, 0.0f, 0.0f), d = o;
+        uint32_t ti = 0, n_prims = 0, n_nodes = 0;
+        float r0 = 0.0f;
+        if (got) {
+          k = chk(slot_of(pl, a.seg_cap, mine, seg), a.slots, K_SLOT, a.ctl);
+          uint32_t tbk = 0;
+          load_ray<true>(q, k, &o, &d, &tbk);
+          ti = chk(tbk >> 8, a.ticket_cap, K_TICKET, a.ctl);
+        }
+        uint64_t todo = __ballot(got);
+        while (todo) {
+          const int leader = __ffsll((unsigned long long)todo) - 1;
+          const uint32_t t = (uint32_t)__builtin_amdgcn_readlane((int)ti, leader);
+          // membership from the ballot's bit: under ti == t the compiler
+          // would read the tables at ti again
+          const uint64_t same = __ballot(got && ti == t);
+          todo &= ~same;
+          if ((same >> lane_id()) & 1ull) {
+            const DevTicket T = load_constant(a.tickets + t);
+            const DevJob J = load_constant(a.jobs + chk(T.job, a.job_cap, K_JOB, a.ctl));
+            n_prims = J.n_prims;
+            n_nodes = J.n_nodes;
+            r0 = J.r0_tol;
+            nodes = J.nodes;
+            nodes4 = J.nodes4;
+            prims = J.prims;
+#ifdef RDH_CHECKED
+            cur_nodes = W4 ? J.n_nodes4 : J.n_nodes;
+            cur_prims = J.n_prims;
+#endif
+          }
+        }
+        if (got) start(o, d, n_prims, n_nodes, r0);
       }
     }
     if (__ballot(active) == 0) {
