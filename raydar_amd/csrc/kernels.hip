@@ -142,6 +142,20 @@ __device__ __forceinline__ PathQueue queue_of(const IterArgs &a, int p) {
   return p ? a.q[1] : a.q[0];
 }
 
+// A record of a table that no kernel writes during a launch, read through
+// the constant address space: at a wave-uniform address that is a scalar
+// load.  The pointer is rebuilt from its integer value, which the compiler
+// cannot fold back to a global pointer (a plain cast it folds, and the loads
+// stay vector loads).  The host pass of the compiler only checks the body.
+template <class T>
+__device__ __forceinline__ T load_constant(const T *p) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return *(const __attribute__((address_space(4))) T *)(uintptr_t)p;
+#else
+  return *p;
+#endif
+}
+
 // Largest ticket index in [lo, n) whose work_begin <= w (tickets sorted by
 // work_begin; the host keeps lo at or below the ticket holding the cursor).
 __device__ __forceinline__ uint32_t find_ticket(const DevTicket *t, uint32_t lo, uint32_t n,
@@ -150,6 +164,17 @@ __device__ __forceinline__ uint32_t find_ticket(const DevTicket *t, uint32_t lo,
   while (hi - lo > 1) {
     const uint32_t mid = (lo + hi) >> 1;
     if (t[mid].work_begin <= w) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+// The same search at a wave-uniform work index: every probe is a scalar load.
+__device__ __forceinline__ uint32_t find_ticket_uniform(const DevTicket *t, uint32_t lo, uint32_t n,
+                                                        unsigned long long w) {
+  uint32_t hi = n;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (load_constant(&t[mid].work_begin) <= w) lo = mid;
     else hi = mid;
   }
   return lo;
@@ -169,19 +194,6 @@ typedef __attribute__((address_space(1))) const VecF2 GlobalF2;
 typedef __attribute__((address_space(1))) const VecI4 GlobalI4;
 typedef __attribute__((address_space(1))) const VecU4 GlobalU4;
 typedef __attribute__((address_space(1))) const VecU2 GlobalU2;
-// A record of a table that no kernel writes during a launch, read through
-// the constant address space: at a wave-uniform address that is a scalar
-// load.  The pointer is rebuilt from its integer value, which the compiler
-// cannot fold back to a global pointer (a plain cast it folds, and the loads
-// stay vector loads).  The host pass of the compiler only checks the body.
-template <class T>
-__device__ __forceinline__ T load_constant(const T *p) {
-#if defined(__HIP_DEVICE_COMPILE__)
-  return *(const __attribute__((address_space(4))) T *)(uintptr_t)p;
-#else
-  return *p;
-#endif
-}
 typedef __attribute__((address_space(1))) VecF4 GlobalF4Out;
 typedef __attribute__((address_space(1))) VecF2 GlobalF2Out;
 typedef __attribute__((address_space(1))) VecU4 GlobalU4Out;
@@ -337,6 +349,16 @@ __device__ __forceinline__ uint32_t ticket_of(uint32_t tb) { return SINGLE ? 0u 
 template <bool COUNT, bool SINGLE, bool VK = false>
 __global__ void k_generate(IterArgs a, int p) {
   __shared__ SegPlan pl;
+  // One job (SINGLE): its camera and its one ticket are read here, before
+  // any global store, as scalar loads kept in scalar registers.  Read inside
+  // the loop, after the first stores, they would be vector loads and a
+  // memory round trip per iteration; a camera path needs no other read.
+  DevJob job0;
+  DevTicket ticket0;
+  if (SINGLE) {
+    job0 = a.jobs[0];
+    ticket0 = a.tickets[0];
+  }
   make_plan(a, p, pl);
   const uint32_t stride = gridDim.x * blockDim.x;
   const PathQueue q = queue_of(a, p);
@@ -346,29 +368,49 @@ __global__ void k_generate(IterArgs a, int p) {
     const unsigned long long w = pl.cursor + i;
     const uint32_t s = segment_of(pl.pre_new, i);
     const uint32_t k = chk(s * a.seg_cap + pl.cnt[s] + (i - pl.pre_new[s]), a.slots, K_SLOT, a.ctl);
-    uint32_t ti = 0;
-    if (!SINGLE) {
+    uint32_t ti = 0, pidx = 0;
+    F3 o, d;
+    auto camera = [&](const DevJob &J, const DevTicket &T) {
+      pidx = (uint32_t)(w - T.work_begin);
+      const uint32_t npix = J.width * J.height;
+      const uint32_t pixel = pidx % npix;
+      if (VK) {
+        const uint32_t x = pixel % J.width, y = pixel / J.width;
+        camera_ray_vk(J, x, y, &o, &d);
+        if (T.sample_base == 0) J.vk_rng[chk(pixel, npix, K_RADIANCE, a.ctl)] = x + y * 1000u;
+      } else {
+        camera_ray(J, pixel % J.width, pixel / J.width, &o, &d);
+      }
+    };
+    if (SINGLE) {
+      camera(job0, ticket0);
+    } else {
       // the wave's lanes hold consecutive work indices: one search for the
-      // first active lane's index (uniform, scalar loads) serves nearly all
+      // first active lane's index (wave-uniform: scalar loads) serves nearly
+      // all; lanes past that ticket's end search again on their own
       const unsigned long long w0 =
           ((unsigned long long)__builtin_amdgcn_readfirstlane((uint32_t)(w >> 32)) << 32) |
           __builtin_amdgcn_readfirstlane((uint32_t)w);
-      ti = find_ticket(a.tickets, t_lo, t_n, w0);
-      if (w >= a.tickets[ti].work_begin + a.tickets[ti].paths) ti = find_ticket(a.tickets, ti, t_n, w);
-    }
-    ti = chk(ti, a.ticket_cap, K_TICKET, a.ctl);
-    const DevTicket &T = a.tickets[ti];
-    const DevJob &J = a.jobs[SINGLE ? 0u : chk(T.job, a.job_cap, K_JOB, a.ctl)];
-    const uint32_t pidx = (uint32_t)(w - T.work_begin);
-    const uint32_t npix = J.width * J.height;
-    const uint32_t pixel = pidx % npix;
-    F3 o, d;
-    if (VK) {
-      const uint32_t x = pixel % J.width, y = pixel / J.width;
-      camera_ray_vk(J, x, y, &o, &d);
-      if (T.sample_base == 0) J.vk_rng[chk(pixel, npix, K_RADIANCE, a.ctl)] = x + y * 1000u;
-    } else {
-      camera_ray(J, pixel % J.width, pixel / J.width, &o, &d);
+      ti = find_ticket_uniform(a.tickets, t_lo, t_n, w0);
+      const DevTicket t0 = load_constant(a.tickets + ti);
+      if (w >= t0.work_begin + t0.paths) ti = find_ticket(a.tickets, ti, t_n, w);
+      ti = chk(ti, a.ticket_cap, K_TICKET, a.ctl);
+      // one round per distinct ticket among the lanes (nearly always one),
+      // with the ticket and job records read as scalar loads
+      uint64_t todo = __ballot(true);
+      while (todo) {
+        const int leader = __ffsll((unsigned long long)todo) - 1;
+        const uint32_t t = (uint32_t)__builtin_amdgcn_readlane((int)ti, leader);
+        // membership from the ballot's bit: under ti == t the compiler
+        // would read the tables at ti again
+        const uint64_t same = __ballot(ti == t);
+        todo &= ~same;
+        if ((same >> lane_id()) & 1ull) {
+          const DevTicket T = load_constant(a.tickets + t);
+          const DevJob J = load_constant(a.jobs + chk(T.job, a.job_cap, K_JOB, a.ctl));
+          camera(J, T);
+        }
+      }
     }
     // attenuation and light of a camera path are (1,1,1) and 0 (cpu.rs:253-254):
     // shade uses the constants at bounce 0, so they are not stored
