@@ -343,6 +343,33 @@ template <bool SINGLE>
 __device__ __forceinline__ uint32_t ticket_of(uint32_t tb) { return SINGLE ? 0u : (tb >> 8); }
 
 // ------------------------------------------------------------ generate ----
+// Pixel of the r-th camera path of a sample: 8x8 tiles, row by row inside a
+// tile and tile by tile along each band of 8 rows (the columns right of the
+// last whole tile row-major inside the band; a last band of fewer than 8 rows
+// row-major), a permutation of [0, W*H).  A wave's 64 consecutive paths then
+// start from one 8x8 tile instead of 64 pixels of one row: their rays and,
+// after a bounce, their origins stay close together, which the closest-hit
+// kernel's node fetches and traversal order profit from.  A path's draws,
+// radiance slot and accumulation are keyed by its pixel, so the order changes
+// no result.
+__device__ __forceinline__ uint32_t tile_order_pixel(uint32_t r, uint32_t W, uint32_t H) {
+  const uint32_t band_px = 8u * W;
+  const uint32_t band = r / band_px;
+  if ((band + 1u) * 8u > H) return r;
+  const uint32_t q = r - band * band_px;
+  const uint32_t full = W & ~7u;
+  uint32_t x, y;
+  if (q < full * 8u) {
+    x = (q >> 6) * 8u + (q & 7u);
+    y = band * 8u + ((q >> 3) & 7u);
+  } else {
+    const uint32_t e = q - full * 8u, rw = W - full;
+    x = full + e % rw;
+    y = band * 8u + e / rw;
+  }
+  return y * W + x;
+}
+
 // VK (Vulkan semantics, one sample per pass): the pinhole camera of
 // raytrace.rgen:83-89; the pass of sample 0 also seeds each pixel's PCG
 // state with x + y * 1000 (raytrace.rgen:50).
@@ -371,9 +398,11 @@ __global__ void k_generate(IterArgs a, int p) {
     uint32_t ti = 0, pidx = 0;
     F3 o, d;
     auto camera = [&](const DevJob &J, const DevTicket &T) {
-      pidx = (uint32_t)(w - T.work_begin);
+      const uint32_t j = (uint32_t)(w - T.work_begin);
       const uint32_t npix = J.width * J.height;
-      const uint32_t pixel = pidx % npix;
+      const uint32_t smp = j / npix;
+      const uint32_t pixel = tile_order_pixel(j - smp * npix, J.width, J.height);
+      pidx = smp * npix + pixel;
       if (VK) {
         const uint32_t x = pixel % J.width, y = pixel / J.width;
         camera_ray_vk(J, x, y, &o, &d);
