@@ -59,6 +59,26 @@ def test_plain_small_matches_oracle(opts):
     np.testing.assert_array_equal(img.reshape(-1), oracle.tonemap(ref, 16))
 
 
+@pytest.mark.parametrize("size", [(37, 29), (5, 3), (61, 8), (16, 17)])
+def test_image_sizes_off_the_tile_grid_match_oracle(size):
+    # camera paths are generated in 8x8 pixel tiles; a width that is not a
+    # multiple of 8 leaves a column strip and a height a last band of fewer
+    # than 8 rows, each taken row-major: every pixel must still get exactly
+    # its own samples
+    sc = scenegen.plain_1080p(1, *size)
+    print(f"frame {size[0]}x{size[1]}", flush=True)
+    img, acc = _gpu_acc(sc, 3, 12, 7)
+    ref = oracle.render(sc, 3, 12, 7)
+    np.testing.assert_array_equal(acc.view(np.uint32), ref.view(np.uint32))
+    np.testing.assert_array_equal(img.reshape(-1), oracle.tonemap(ref, 3))
+    with ra.HipRenderer(max_sample_count=3, max_bounces=12, seed=7, semantics=1) as r:
+        img_vk = r.render_frame(sc)
+        acc_vk = r.accumulator()
+    ref_vk = oracle.vk_render(sc, ra.camera_vulkan_inverse_view(sc.camera), 3, 12, threads=8)
+    np.testing.assert_array_equal(acc_vk.view(np.uint32), ref_vk.view(np.uint32))
+    np.testing.assert_array_equal(img_vk.reshape(-1), oracle.vk_tonemap(ref_vk, 3))
+
+
 @pytest.mark.parametrize("bvh4", [0, 1])
 @pytest.mark.parametrize("lds_bvh", [0, 14336, 1 << 17])
 def test_lds_resident_bvh_matches_oracle(lds_bvh, bvh4):
