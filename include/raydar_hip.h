@@ -258,7 +258,8 @@ rd_status rd_get_option(const rd_renderer *r, const char *key, int64_t *value);
 /* Algorithmic event counters collected while option "count" was 1. */
 typedef struct rd_counters {
   uint64_t camera_rays;    /* paths started                               */
-  uint64_t segments;       /* closest-hit queries (rays), including misses */
+  uint64_t segments;       /* closest-hit queries (rays traced), including
+                              misses                                       */
   uint64_t box_tests;      /* BVH child-box slab tests                     */
   uint64_t sphere_tests;   /* hit_sphere evaluations (cpu.rs:34-62)        */
   uint64_t cube_tests;     /* hit_cube evaluations (cpu.rs:64-98)          */
@@ -276,6 +277,11 @@ typedef struct rd_counters {
   uint64_t prim_trips;     /* wave iterations of the primitive loop        */
   uint64_t cube_trips;     /* those in which a lane tested a cube          */
   uint64_t refills;        /* times a wave handed new rays to idle lanes   */
+  /* camera paths that took their pixel's closest hit, recorded by an earlier
+     path of the same pass, instead of a query (a pixel's camera ray is the
+     same for every sample); sphere_hits + cube_hits + misses equals
+     segments + reused_hits                                                  */
+  uint64_t reused_hits;
 } rd_counters;
 rd_status rd_get_counters(const rd_renderer *r, rd_counters *out);
 rd_status rd_reset_counters(rd_renderer *r);

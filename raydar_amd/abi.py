@@ -91,7 +91,7 @@ class RdCounters(C.Structure):
         "camera_rays", "segments", "box_tests", "sphere_tests", "cube_tests",
         "sphere_hits", "cube_hits", "glass_scatters", "misses", "accumulates",
         "tonemapped_px", "wave_steps", "lane_steps", "sort_steps", "prim_trips", "cube_trips",
-        "refills")]
+        "refills", "reused_hits")]
 
 
 class RdKernelTimes(C.Structure):

@@ -82,6 +82,8 @@ void Engine::release_buffers() {
   if (hit_t) { (void)hipFree(hit_t); hit_t = nullptr; }
   if (hit_id) { (void)hipFree(hit_id); hit_id = nullptr; }
   if (L) { (void)hipFree(L); L = nullptr; }
+  if (first_hit_) { (void)hipFree(first_hit_); first_hit_ = nullptr; }
+  first_hit_px_ = 0;
   q_alloc = 0;
   l_alloc = 0;
   if (d_tickets) { (void)hipFree(d_tickets); d_tickets = nullptr; }
@@ -162,7 +164,7 @@ bool Engine::ensure_buffers(std::string *err) {
 uint64_t Engine::device_bytes() const {
   // two queues of 14 words per slot plus the 8-byte hit record
   uint64_t b = (uint64_t)kSegments * seg_cap * (2 * 56 + 8) + l_alloc * sizeof(float4) +
-               (uint64_t)ticket_alloc * (sizeof(DevTicket) + 8);
+               (uint64_t)ticket_alloc * (sizeof(DevTicket) + 8) + first_hit_px_ * sizeof(uint2);
   for (const auto &e : aux_) b += e->device_bytes();
   return b;
 }
@@ -332,6 +334,8 @@ IterArgs Engine::iter_args(const DevJob *d_jobs) const {
   a.bvh4 = 0;
   a.leaf1 = 0;
   a.vk = opt.vk;
+  a.first_hit = nullptr;
+  a.first_hit_mode = 0;
   a.chunk_log2 = opt.lane_chunk_log2;
   a.slots = (uint32_t)kSegments * seg_cap;
   a.ticket_cap = ticket_alloc;
@@ -584,11 +588,34 @@ bool Engine::run_tickets(const DevJob *d_jobs, std::vector<DevTicket> tickets,
       const uint64_t cap_iters = (wk / e->q_alloc + 2) * (uint64_t)(max_bounces + 1) + 64;
       pipes.push_back(Pipe{e, a, wk, 0, 0, cap_iters, false, kNoBound});
     }
+    // Camera-ray hits: one scene, CPU semantics and an owned queue that holds
+    // a whole sample, so that each pipe's first iteration generates every
+    // pixel's camera path at least once and its shade records each pixel's
+    // hit; from the second iteration on, fresh paths take their pixel's hit
+    // and extend traces only the survivors (same hits, same bits: the camera
+    // ray of a pixel does not depend on the sample).
+    bool first_hits = !external_ && !opt.vk && host_jobs.size() == 1 && pipes[0].a.single_job &&
+                      !opt.brute_force;
+    uint64_t npix_fh = first_hits ? (uint64_t)host_jobs[0].width * host_jobs[0].height : 0;
+    for (const Pipe &p : pipes) first_hits = first_hits && p.e->q_alloc >= npix_fh && p.w >= npix_fh;
+    if (first_hits && npix_fh > first_hit_px_) {
+      if (first_hit_) (void)hipFree(first_hit_);
+      first_hit_ = nullptr;
+      first_hit_px_ = 0;
+      if (!check(hipMalloc(&first_hit_, sizeof(uint2) * npix_fh), "hipMalloc(first hits)", err))
+        return false;
+      first_hit_px_ = npix_fh;
+    }
+    for (Pipe &p : pipes) p.a.first_hit = first_hits ? first_hit_ : nullptr;
     const int group = std::max(1, opt.sync_every);
     for (;;) {
       for (int g = 0; g < group; ++g)
         for (Pipe &p : pipes) {
           if (p.done) continue;
+          p.a.first_hit_mode = first_hits ? (p.it == 0 ? 1 : 2) : 0;
+          // the recording iteration generates one sample only: the pipe's
+          // other camera paths all take their hits from it
+          p.a.capacity = first_hits && p.it == 0 ? (uint32_t)npix_fh : p.e->q_alloc;
           if (!p.e->launch_iteration(p.a, p.parity, err, p.bound)) return false;
           p.parity ^= 1;
           ++p.it;

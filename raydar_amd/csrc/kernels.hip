@@ -51,7 +51,11 @@ struct SegPlan {
   unsigned long long cursor;
 };
 
-__device__ __forceinline__ void make_plan(const IterArgs &a, int p, SegPlan &sp) {
+// survivors_only: the consumers' order covers only the survivors (cnt[s] of
+// each segment, n_old in all), for an extend whose fresh paths already have
+// their hits (first_hit_mode 2); n_new and the cursor are unchanged.
+__device__ __forceinline__ void make_plan(const IterArgs &a, int p, SegPlan &sp,
+                                          bool survivors_only = false) {
   if (threadIdx.x < kSegments) {
     // a counter above seg_cap only follows an overflow (already flagged for
     // the host); clamping keeps every slot index of this iteration in bounds
@@ -75,7 +79,7 @@ __device__ __forceinline__ void make_plan(const IterArgs &a, int p, SegPlan &sp)
       left -= add;
       sp.add[s] = add;
       sp.pre_new[s + 1] = sp.pre_new[s] + add;
-      sp.pre[s + 1] = sp.pre[s] + sp.cnt[s] + add;
+      sp.pre[s + 1] = sp.pre[s] + sp.cnt[s] + (survivors_only ? 0u : add);
     }
     sp.n_new = n_new;
     sp.n = sp.pre[kSegments];
@@ -390,18 +394,18 @@ __global__ void k_generate(IterArgs a, int p) {
   const uint32_t stride = gridDim.x * blockDim.x;
   const PathQueue q = queue_of(a, p);
   const uint32_t t_lo = a.ctl->ticket_lo, t_n = a.ctl->n_tickets;
-  uint32_t made = 0;
+  uint32_t made = 0, reused = 0;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < pl.n_new; i += stride) {
     const unsigned long long w = pl.cursor + i;
     const uint32_t s = segment_of(pl.pre_new, i);
     const uint32_t k = chk(s * a.seg_cap + pl.cnt[s] + (i - pl.pre_new[s]), a.slots, K_SLOT, a.ctl);
-    uint32_t ti = 0, pidx = 0;
+    uint32_t ti = 0, pidx = 0, pixel = 0;
     F3 o, d;
     auto camera = [&](const DevJob &J, const DevTicket &T) {
       const uint32_t j = (uint32_t)(w - T.work_begin);
       const uint32_t npix = J.width * J.height;
       const uint32_t smp = j / npix;
-      const uint32_t pixel = tile_order_pixel(j - smp * npix, J.width, J.height);
+      pixel = tile_order_pixel(j - smp * npix, J.width, J.height);
       pidx = smp * npix + pixel;
       if (VK) {
         const uint32_t x = pixel % J.width, y = pixel / J.width;
@@ -413,6 +417,14 @@ __global__ void k_generate(IterArgs a, int p) {
     };
     if (SINGLE) {
       camera(job0, ticket0);
+      if (a.first_hit_mode == 2) {
+        // the pixel's camera-ray hit, recorded by this pass's first shade:
+        // extend leaves this fresh path alone
+        const uint2 h = a.first_hit[chk(pixel, job0.width * job0.height, K_RADIANCE, a.ctl)];
+        a.hit_t[k] = bitsf(h.x);
+        a.hit_id[k] = h.y;
+        if (COUNT) ++reused;
+      }
     } else {
       // the wave's lanes hold consecutive work indices: one search for the
       // first active lane's index (wave-uniform: scalar loads) serves nearly
@@ -447,6 +459,7 @@ __global__ void k_generate(IterArgs a, int p) {
     ++made;
   }
   if (COUNT && made) atomicAdd(&a.counters->v[C_CAMERA], (unsigned long long)made);
+  if (COUNT && reused) atomicAdd(&a.counters->v[C_REUSED], (unsigned long long)reused);
 }
 
 // -------------------------------------------------------------- extend ----
@@ -455,7 +468,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(RDH_EXTEN
 void k_extend(IterArgs a, int p) {
   __shared__ StackT stack[kStackDepth * BLOCK];
   __shared__ SegPlan pl;
-  make_plan(a, p, pl);
+  make_plan(a, p, pl, a.first_hit_mode == 2);
   // queue[1-p] is empty until shade fills it; nothing else reads it now
   if (blockIdx.x == 0 && threadIdx.x < kSegments) a.ctl->count[1 - p][threadIdx.x * kSegStride] = 0;
   if (blockIdx.x == 0 && threadIdx.x == 0) a.ctl->cursor[1 - p] = pl.cursor + pl.n_new;
@@ -550,7 +563,7 @@ void k_extend_lanes(IterArgs a, int p) {
     for (uint32_t i = threadIdx.x; i < node_vecs; i += BLOCK) lds_nodes[i] = gn[i];
     for (uint32_t i = threadIdx.x; i < 2 * J.n_prims; i += BLOCK) lds_prims[i] = gp[i];
   }
-  make_plan(a, p, pl);  // its barriers also publish the LDS copy
+  make_plan(a, p, pl, a.first_hit_mode == 2);  // its barriers also publish the LDS copy
   // queue[1-p] is empty until shade fills it; nothing else reads it now
   if (blockIdx.x == 0 && threadIdx.x < kSegments) a.ctl->count[1 - p][threadIdx.x * kSegStride] = 0;
   if (blockIdx.x == 0 && threadIdx.x == 0) a.ctl->cursor[1 - p] = pl.cursor + pl.n_new;
@@ -1048,6 +1061,11 @@ void k_shade(IterArgs a, int p) {
     // the path's bounce with its ticket T and job J
     auto shade_path = [&](const DevJob &J, const DevTicket &T) {
       l_slot = chk((unsigned long long)(T.l_offset + pidx), a.l_cap, K_RADIANCE, a.ctl);
+      if (SINGLE && a.first_hit_mode == 1 && bounce == 0) {
+        // a camera path's hit is its pixel's for every sample of the pass
+        const uint32_t npix = J.width * J.height;
+        a.first_hit[chk(pidx % npix, npix, K_RADIANCE, a.ctl)] = make_uint2(fbits(hit_t), id);
+      }
       if (id != kNoHit) {
         const uint32_t oid = chk(id, J.n_prims, K_OBJECT, a.ctl);
         const DevObject ob = lds_obj ? object_from(lds_objects[4 * oid], lds_objects[4 * oid + 1],

@@ -172,12 +172,12 @@ struct QueueCtl {
 
 // Algorithmic event counters (collected only by the counting build).
 struct DevCounters {
-  unsigned long long v[17];
+  unsigned long long v[18];
 };
 enum CounterIndex {
   C_CAMERA = 0, C_SEGMENTS, C_BOX, C_SPHERE, C_CUBE, C_SPHERE_HIT, C_CUBE_HIT,
   C_GLASS, C_MISS, C_ACCUM, C_TONEMAP,
-  C_WAVE_STEPS, C_LANE_STEPS, C_SORT_STEPS, C_PRIM_TRIPS, C_CUBE_TRIPS, C_REFILLS
+  C_WAVE_STEPS, C_LANE_STEPS, C_SORT_STEPS, C_PRIM_TRIPS, C_CUBE_TRIPS, C_REFILLS, C_REUSED
 };
 
 }  // namespace rdh

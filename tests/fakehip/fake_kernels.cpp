@@ -139,6 +139,21 @@ bool check_args(const IterArgs &a, const LaunchShape &sh, const char *k) {
   ok = need(a.jobs, sizeof(DevJob), k, "job table") && ok;
   if (a.L) ok = need(a.L, 16, k, "radiance arena") && ok;
   if (a.track_done) ok = need(a.ticket_done, 8ull * ticket_cap(a), k, "ticket progress") && ok;
+  if (a.first_hit_mode != 0) {
+    // camera-ray hits: one single-job scene's pixels, whose camera paths the
+    // first iteration must all generate (the queue holds a whole sample)
+    if (!a.single_job || a.vk || a.first_hit_mode > 2) {
+      err(k, "camera-ray hits outside a single-job CPU-semantics pass");
+      ok = false;
+    } else if (need(a.jobs, sizeof(DevJob), k, "job table")) {
+      const uint64_t npix = (uint64_t)a.jobs[0].width * a.jobs[0].height;
+      ok = need(a.first_hit, 8ull * npix, k, "camera-ray hits") && ok;
+      if (a.capacity < npix) {
+        err(k, "camera-ray hits with a queue smaller than one sample");
+        ok = false;
+      }
+    }
+  }
   return ok;
 }
 

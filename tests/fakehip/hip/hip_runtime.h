@@ -74,6 +74,9 @@ struct hipDeviceProp_t {
 struct float4 {
   float x, y, z, w;
 };
+struct uint2 {
+  unsigned int x, y;
+};
 struct uint4 {
   unsigned int x, y, z, w;
 };

@@ -373,6 +373,29 @@ def test_wide_stack_scene_matches_oracle():
     np.testing.assert_array_equal(job.acc.view(np.uint32), ref.view(np.uint32))
 
 
+@pytest.mark.parametrize("opts,reuse_pipes", [(dict(), 1), (dict(pipes=2, queue=16384), 2),
+                                              (dict(queue=4096), None),
+                                              (dict(lds_bvh=0, bvh4=0), 1)])
+def test_camera_ray_hits_reused_after_each_pipes_first_sample(opts, reuse_pipes):
+    # a pixel's camera ray does not depend on the sample (cpu.rs:234-251): each
+    # pipe traces one sample's camera rays and gives its later camera paths
+    # their pixel's recorded hit; a queue smaller than one sample traces all
+    sc = scenegen.plain_1080p(1, 96, 54)
+    npix, spp = sc.width * sc.height, 8
+    with ra.HipRenderer(max_sample_count=spp, max_bounces=12, seed=4, **opts) as r:
+        r.reset_counters()
+        r.set_option("count", 1)
+        r.render_frame(sc)
+        c = r.counters()
+        acc = r.accumulator()
+    assert c["camera_rays"] == spp * npix
+    want = 0 if reuse_pipes is None else (spp - reuse_pipes) * npix
+    assert c["reused_hits"] == want
+    assert c["sphere_hits"] + c["cube_hits"] + c["misses"] == c["segments"] + c["reused_hits"]
+    ref = oracle.render(sc, spp, 12, 4)
+    np.testing.assert_array_equal(acc.view(np.uint32), ref.view(np.uint32))
+
+
 def test_event_counts_agree_between_trees():
     # the counting pass behind the roofline figure: the two trees give the
     # same paths, so every per-path event agrees and only the traversal
@@ -389,7 +412,10 @@ def test_event_counts_agree_between_trees():
     assert a["camera_rays"] == b["camera_rays"] == sc.width * sc.height * 2
     for k in ("sphere_hits", "cube_hits", "glass_scatters", "misses", "accumulates"):
         assert a[k] == b[k], k
-    assert a["sphere_hits"] + a["cube_hits"] + a["misses"] == a["segments"] == b["segments"]
+    # every shaded path got its hit from a query or from its pixel's camera
+    # ray (the second sample of this 2-spp frame reuses the first's)
+    assert a["segments"] == b["segments"] and a["reused_hits"] == b["reused_hits"]
+    assert a["sphere_hits"] + a["cube_hits"] + a["misses"] == a["segments"] + a["reused_hits"]
     for k in ("box_tests", "sphere_tests", "cube_tests"):
         assert a[k] > 0 and b[k] > 0, k
     # the lanes kernel's SIMD-efficiency counts obey their definitions
