@@ -168,7 +168,8 @@ def batched_vs_plain(reps=2, gap=0.0, cpu=False, opts=None):
                                                                     "solo_paths")},
                           "solo_jobs": br.get_option("solo_jobs")}
     solo_paths = out["batched"]["options"]["solo_paths"]
-    with ra.HipRenderer(**opts) as r:
+    # the solo-lane threshold is the batch renderer's own option
+    with ra.HipRenderer(**{k: v for k, v in opts.items() if k != "solo_paths"}) as r:
         runs = []
         large_s = [0.0] * (reps + 1)  # time of the jobs the batch gives the solo lane
         for k in range(reps + 1):  # the first pass warms up, as for the batch
