@@ -58,7 +58,7 @@ void rd_batch_destroy(rd_batch *b);
  * "bvh4" (1 = 4-wide trees while every resident job's walk fits the
  * kernel's stack, the default; 0 = binary trees),
  * "graph" (1 = replay a captured hipGraph),
- * "l_capacity" (radiance slots, 2^16..2^31), "iters_per_pump" (even, >= 2),
+ * "l_capacity" (radiance slots, 2^16..2^30), "iters_per_pump" (even, >= 2),
  * "ticket_cap" (passes in the work list, default 2^20, fixed once rendering
  * started: the list starts over whenever the engine has drained, and at the
  * cap new passes wait for that), read-only "rewinds" (first pipe's count),

@@ -699,7 +699,7 @@ rd_status rd_set_option(rd_renderer *r, const char *key, int64_t v) {
   } else if (k == "kernel_timing") {
     o.kernel_timing = v != 0;
   } else if (k == "l_capacity") {
-    if (v < (1 << 16) || v > (1ll << 31)) return r->fail(RD_ERR_INVALID_ARGUMENT, "l_capacity out of range");
+    if (v < (1 << 16) || v > (1ll << 30)) return r->fail(RD_ERR_INVALID_ARGUMENT, "l_capacity out of range");
     o.l_capacity = (uint64_t)v;
   } else if (k == "sync_every") {
     if (v < 1 || v > 1024) return r->fail(RD_ERR_INVALID_ARGUMENT, "sync_every out of range");

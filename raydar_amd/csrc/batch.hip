@@ -207,7 +207,7 @@ class Streamer {
     j->out_acc = d.accumulator;
     j->progress = d.progress_rgba;
     j->bytes = j->built.nodes.size() * sizeof(DevNode) + j->built.prims.size() * sizeof(DevPrim) +
-               j->built.objects.size() * sizeof(DevObject) + npix * 20;
+               j->built.objects.size() * sizeof(DevObject) + npix * 28;
     j->t_submit = Clock::now();
     return j;
   }
@@ -343,12 +343,15 @@ class Streamer {
       free_slots_.pop_back();
       const uint64_t npix = (uint64_t)j->W * j->H;
       if (!eng.upload_scene(j->built, &j->dev, &err)) return false;
-      if (!eng.check(eng.dmalloc((void **)&j->acc, 16 * npix), "hipMalloc(accumulator)", &err)) return false;
+      // the accumulator (16 B per pixel) is followed by the job's camera-ray
+      // hit table (8 B per pixel), zero (not recorded) like the accumulator
+      if (!eng.check(eng.dmalloc((void **)&j->acc, 24 * npix), "hipMalloc(accumulator)", &err)) return false;
       if (!eng.check(eng.dmalloc((void **)&j->rgba_d, 4 * npix), "hipMalloc(image)", &err)) return false;
-      if (!eng.check(launch_zero_acc(j->acc, 4 * npix, eng.stream), "zero accumulator", &err)) return false;
+      if (!eng.check(launch_zero_acc(j->acc, 6 * npix, eng.stream), "zero accumulator", &err)) return false;
       // the host mirror entry outlives the asynchronous copy
       DevJob &dj = h_jobs_[j->slot];
       fill_job(j->scene, j->dev, j->acc, j->bounces, j->seed, &dj);
+      dj.first_hit = reinterpret_cast<HitWord *>(j->acc + 4 * npix);
       if (!eng.check(hipMemcpyAsync(d_jobs_ + j->slot, &dj, sizeof(DevJob), hipMemcpyHostToDevice,
                                     eng.stream),
                      "upload job", &err))
@@ -1093,7 +1096,7 @@ bool engine_option(rdh::Streamer &st, const std::string &k, int64_t v, std::stri
     if (v < 2 || v > 1024) { *err = "iters_per_pump out of range"; return false; }
     st.iters_per_pump = (int)v;
   } else if (k == "l_capacity") {
-    if (v < (1 << 16) || v > (1ll << 31)) { *err = "l_capacity out of range"; return false; }
+    if (v < (1 << 16) || v > (1ll << 30)) { *err = "l_capacity out of range"; return false; }
     if (st.eng.l_alloc && (uint64_t)v != st.eng.l_alloc) { *err = "l_capacity is fixed once rendering started"; return false; }
     o.l_capacity = (uint64_t)v;
   } else if (k == "traversal") {

@@ -164,7 +164,7 @@ bool Engine::ensure_buffers(std::string *err) {
 uint64_t Engine::device_bytes() const {
   // two queues of 14 words per slot plus the 8-byte hit record
   uint64_t b = (uint64_t)kSegments * seg_cap * (2 * 56 + 8) + l_alloc * sizeof(float4) +
-               (uint64_t)ticket_alloc * (sizeof(DevTicket) + 8) + first_hit_px_ * sizeof(uint2);
+               (uint64_t)ticket_alloc * (sizeof(DevTicket) + 8) + first_hit_px_ * sizeof(HitWord);
   for (const auto &e : aux_) b += e->device_bytes();
   return b;
 }
@@ -602,7 +602,7 @@ bool Engine::run_tickets(const DevJob *d_jobs, std::vector<DevTicket> tickets,
       if (first_hit_) (void)hipFree(first_hit_);
       first_hit_ = nullptr;
       first_hit_px_ = 0;
-      if (!check(hipMalloc(&first_hit_, sizeof(uint2) * npix_fh), "hipMalloc(first hits)", err))
+      if (!check(hipMalloc(&first_hit_, sizeof(HitWord) * npix_fh), "hipMalloc(first hits)", err))
         return false;
       first_hit_px_ = npix_fh;
     }

@@ -191,7 +191,7 @@ class Engine {
   hipEvent_t arena_free_ = nullptr;           // this stream's work before a pass
   DevTicket *tickets_pin_ = nullptr;          // pinned source of run_tickets' uploads
   uint32_t tickets_pin_cap_ = 0;
-  uint2 *first_hit_ = nullptr;                // per-pixel camera-ray hits (IterArgs::first_hit)
+  HitWord *first_hit_ = nullptr;              // per-pixel camera-ray hits (IterArgs::first_hit)
   uint64_t first_hit_px_ = 0;                 // pixels first_hit_ holds
 };
 

@@ -203,15 +203,19 @@ typedef __attribute__((address_space(1))) VecF2 GlobalF2Out;
 typedef __attribute__((address_space(1))) VecU4 GlobalU4Out;
 typedef __attribute__((address_space(1))) VecU2 GlobalU2Out;
 
-// A queued path's ray (QRay, QDir), and with WORD its ticket/bounce word.
+// A queued path's ray (QRay, QDir), and with WORD its two words.
 template <bool WORD>
 __device__ __forceinline__ void load_ray(const PathQueue &q, uint32_t k, F3 *o, F3 *d,
-                                         uint32_t *tb) {
+                                         uint32_t *tb, uint32_t *pw = nullptr) {
   const VecF4 a = *(GlobalF4 *)(q.ray + k);
   const VecF2 b = *(GlobalF2 *)(q.dir + k);
   *o = f3(a.x, a.y, a.z);
   *d = f3(a.w, b.x, b.y);
-  if (WORD) *tb = *(__attribute__((address_space(1))) const uint32_t *)&q.word[k].tb;
+  if (WORD) {
+    const VecU2 w = *(GlobalU2 *)(q.word + k);
+    *tb = w.x;
+    if (pw) *pw = w.y;
+  }
 }
 // A path's ray and words, stored into slot k.
 __device__ __forceinline__ void store_ray(const PathQueue &q, uint32_t k, F3 o, F3 d, uint32_t tb,
@@ -399,7 +403,7 @@ __global__ void k_generate(IterArgs a, int p) {
     const unsigned long long w = pl.cursor + i;
     const uint32_t s = segment_of(pl.pre_new, i);
     const uint32_t k = chk(s * a.seg_cap + pl.cnt[s] + (i - pl.pre_new[s]), a.slots, K_SLOT, a.ctl);
-    uint32_t ti = 0, pidx = 0, pixel = 0;
+    uint32_t ti = 0, pidx = 0, pixel = 0, pre = 0;
     F3 o, d;
     auto camera = [&](const DevJob &J, const DevTicket &T) {
       const uint32_t j = (uint32_t)(w - T.work_begin);
@@ -420,9 +424,9 @@ __global__ void k_generate(IterArgs a, int p) {
       if (a.first_hit_mode == 2) {
         // the pixel's camera-ray hit, recorded by this pass's first shade:
         // extend leaves this fresh path alone
-        const uint2 h = a.first_hit[chk(pixel, job0.width * job0.height, K_RADIANCE, a.ctl)];
-        a.hit_t[k] = bitsf(h.x);
-        a.hit_id[k] = h.y;
+        const HitWord h = a.first_hit[chk(pixel, job0.width * job0.height, K_RADIANCE, a.ctl)];
+        a.hit_t[k] = bitsf(h.t1 - 1u);
+        a.hit_id[k] = h.id;
         if (COUNT) ++reused;
       }
     } else {
@@ -450,12 +454,23 @@ __global__ void k_generate(IterArgs a, int p) {
           const DevTicket T = load_constant(a.tickets + t);
           const DevJob J = load_constant(a.jobs + chk(T.job, a.job_cap, K_JOB, a.ctl));
           camera(J, T);
+          if (J.first_hit) {
+            // the pixel's camera-ray hit once a shade has recorded it (zero
+            // until then): the path is flagged, and extend leaves it alone
+            const HitWord h = J.first_hit[chk(pixel, J.width * J.height, K_RADIANCE, a.ctl)];
+            if (h.t1 != 0u) {
+              a.hit_t[k] = bitsf(h.t1 - 1u);
+              a.hit_id[k] = h.id;
+              pre = kPreHitFlag;
+              if (COUNT) ++reused;
+            }
+          }
         }
       }
     }
     // attenuation and light of a camera path are (1,1,1) and 0 (cpu.rs:253-254):
     // shade uses the constants at bounce 0, so they are not stored
-    store_ray(q, k, o, d, ti << 8, pidx);
+    store_ray(q, k, o, d, ti << 8, pidx | pre);
     ++made;
   }
   if (COUNT && made) atomicAdd(&a.counters->v[C_CAMERA], (unsigned long long)made);
@@ -479,8 +494,9 @@ void k_extend(IterArgs a, int p) {
   for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < pl.n; i += stride) {
     const uint32_t k = slot_of(pl, a.seg_cap, i, seg);
     F3 o, d;
-    uint32_t tbk = 0;
-    load_ray<!SINGLE>(q, k, &o, &d, &tbk);
+    uint32_t tbk = 0, pwk = 0;
+    load_ray<!SINGLE>(q, k, &o, &d, &tbk, &pwk);
+    if (!SINGLE && (pwk & kPreHitFlag)) continue;  // generate wrote its hit
     const DevJob &J = a.jobs[SINGLE ? 0u : a.tickets[tbk >> 8].job];
     const RayPre r = ray_pre(o, d);
     float bt;
@@ -715,19 +731,22 @@ void k_extend_lanes(IterArgs a, int p) {
         F3 o = f3(0.0f, 0.0f, 0.0f), d = o;
         uint32_t ti = 0, n_prims = 0, n_nodes = 0;
         float r0 = 0.0f;
+        bool go = false;  // a refilled lane with a ray to trace
         if (got) {
           k = chk(slot_of(pl, a.seg_cap, mine, seg), a.slots, K_SLOT, a.ctl);
-          uint32_t tbk = 0;
-          load_ray<true>(q, k, &o, &d, &tbk);
+          uint32_t tbk = 0, pwk = 0;
+          load_ray<true>(q, k, &o, &d, &tbk, &pwk);
           ti = chk(tbk >> 8, a.ticket_cap, K_TICKET, a.ctl);
+          // a camera path whose hit generate wrote stays idle
+          go = (pwk & kPreHitFlag) == 0u;
         }
-        uint64_t todo = __ballot(got);
+        uint64_t todo = __ballot(go);
         while (todo) {
           const int leader = __ffsll((unsigned long long)todo) - 1;
           const uint32_t t = (uint32_t)__builtin_amdgcn_readlane((int)ti, leader);
           // membership from the ballot's bit: under ti == t the compiler
           // would read the tables at ti again
-          const uint64_t same = __ballot(got && ti == t);
+          const uint64_t same = __ballot(go && ti == t);
           todo &= ~same;
           if ((same >> lane_id()) & 1ull) {
             const DevTicket T = load_constant(a.tickets + t);
@@ -744,7 +763,7 @@ void k_extend_lanes(IterArgs a, int p) {
 #endif
           }
         }
-        if (got) start(o, d, n_prims, n_nodes, r0);
+        if (go) start(o, d, n_prims, n_nodes, r0);
       }
     }
     if (__ballot(active) == 0) {
@@ -1021,7 +1040,7 @@ void k_shade(IterArgs a, int p) {
   for (uint32_t base = blockIdx.x * BLOCK; base < pl.n; base += gridDim.x * BLOCK) {
     const uint32_t i = base + threadIdx.x;
     const bool has = i < pl.n;
-    bool alive = false, finished = false;
+    bool alive = false, finished = false, pre = false;
     uint32_t ti = 0, pidx = 0, bounce = 0, id = kNoHit;
     float hit_t = 0.0f;
     F3 o = f3(0, 0, 0), d = o, at = o, li = o;
@@ -1045,6 +1064,7 @@ void k_shade(IterArgs a, int p) {
       bounce = tb & 0xffu;
       const uint32_t pw = wd.y;
       pidx = pw & kPathIndexMask;
+      pre = (pw & kPreHitFlag) != 0u;
       at = f3(1.0f, 1.0f, 1.0f);
       if (bounce != 0) {
         const VecF4 aw = *(GlobalF4 *)(q.att + k);
@@ -1061,10 +1081,12 @@ void k_shade(IterArgs a, int p) {
     // the path's bounce with its ticket T and job J
     auto shade_path = [&](const DevJob &J, const DevTicket &T) {
       l_slot = chk((unsigned long long)(T.l_offset + pidx), a.l_cap, K_RADIANCE, a.ctl);
-      if (SINGLE && a.first_hit_mode == 1 && bounce == 0) {
-        // a camera path's hit is its pixel's for every sample of the pass
+      if (bounce == 0 && (SINGLE ? a.first_hit_mode == 1 : (J.first_hit != nullptr && !pre))) {
+        // a camera path's hit is its pixel's for every sample (distance bits
+        // + 1, so that zero means not recorded)
         const uint32_t npix = J.width * J.height;
-        a.first_hit[chk(pidx % npix, npix, K_RADIANCE, a.ctl)] = make_uint2(fbits(hit_t), id);
+        HitWord *fh = SINGLE ? a.first_hit : J.first_hit;
+        fh[chk(pidx % npix, npix, K_RADIANCE, a.ctl)] = HitWord{fbits(hit_t) + 1u, id};
       }
       if (id != kNoHit) {
         const uint32_t oid = chk(id, J.n_prims, K_OBJECT, a.ctl);

@@ -54,7 +54,7 @@ struct IterArgs {
                                     // block 256, one sample per pass, no counting)
   // Camera-ray hits reused (single job): a pixel's camera ray is the same for
   // every sample (cpu.rs:234-251 takes no draw), so its closest hit is too.
-  uint2 *first_hit;                 // per pixel: hit distance bits, hit id
+  HitWord *first_hit;               // per pixel
   int first_hit_mode;               // 0 off; 1 shade records each bounce-0 path's hit;
                                     // 2 generate gives fresh paths their pixel's hit
                                     // and extend traces only the survivors

@@ -34,7 +34,11 @@ constexpr uint32_t kNoHit = 0x7fffffffu; // best-id sentinel: "no hit yet"
 // the path's light is not (+0, +0, +0); the light words of a queue slot are
 // read only when it is set (y and z are also written only then).
 constexpr uint32_t kLitFlag = 0x80000000u;
-constexpr uint32_t kPathIndexMask = 0x7fffffffu;
+// a camera path whose hit generate already wrote (several jobs: its job's
+// per-pixel hit table); path indices stay below 2^30 (radiance arenas hold
+// at most 2^30 slots)
+constexpr uint32_t kPreHitFlag = 0x40000000u;
+constexpr uint32_t kPathIndexMask = 0x3fffffffu;
 constexpr int kDrawWords = 8;            // random words per bounce (two Philox blocks)
 
 // Leaf child code: bit 31 set, first prim in bits 4..30, count in bits 0..3.
@@ -85,6 +89,12 @@ struct alignas(16) DevObject {
 };
 static_assert(sizeof(DevObject) == 64, "DevObject must be 64 bytes");
 
+// A pixel's camera-ray hit: distance bits + 1 (0: not recorded yet) and id.
+struct alignas(8) HitWord {
+  uint32_t t1;
+  uint32_t id;
+};
+
 struct DevJob {
   const DevNode *nodes;
   const DevPrim *prims;
@@ -107,6 +117,8 @@ struct DevJob {
   uint32_t max_leaf;        // primitives in the largest leaf (1: one per leaf)
   const DevNode4 *nodes4;
   uint32_t *vk_rng;         // Vulkan semantics: each pixel's PCG state between passes
+  HitWord *first_hit;       // several jobs: each pixel's camera-ray hit, all zero until
+                            // the job's first camera paths are shaded
 };
 
 struct DevTicket {
