@@ -135,6 +135,9 @@ struct rd_renderer {
   uint32_t *vk_rng = nullptr;  // Vulkan semantics: per-pixel PCG state
   uint32_t W = 0, H = 0;
   bool has_frame = false;
+  // bumped by new_frame and restart_frame: the passes of one frame share
+  // their camera-ray hits (recorded by the frame's first pass)
+  uint64_t frame_serial = 0;
   int frame_vk = 0;  // the semantics the current frame was set up with
   rdh::Timer frame_timer, sample_timer, prepare_timer, render_timer;
 
@@ -172,6 +175,7 @@ bool valid_scene(const rd_scene *s, std::string *err) {
 
 // Snapshot a scene into device memory and (re)allocate the accumulator.
 rd_status snapshot(rd_renderer *r, const rd_scene *s, bool zero_acc) {
+  ++r->frame_serial;  // a new scene or camera: its camera-ray hits are recorded anew
   std::string err;
   if (!valid_scene(s, &err)) return r->fail(s && s->world.kind == RD_WORLD_TRANSPARENT
                                                 ? RD_ERR_UNSUPPORTED
@@ -273,6 +277,7 @@ rd_status render_samples(rd_renderer *r, uint32_t k) {
     t.sample_base = r->sample_count + done;
     t.spp = spp;
     t.paths = (uint32_t)(spp * npix);
+    r->eng.hit_frame = r->frame_serial;
     if (!r->eng.run_tickets(r->job_d, {t}, host_jobs, &err)) return device_fail(r, err);
     done += spp;
   }
@@ -389,6 +394,7 @@ rd_status rd_new_frame(rd_renderer *r, const rd_scene *scene) {
 rd_status rd_restart_frame(rd_renderer *r) {
   if (!r) return RD_ERR_INVALID_ARGUMENT;
   if (!r->has_frame) return r->fail(RD_ERR_INVALID_ARGUMENT, "restart_frame before new_frame");
+  ++r->frame_serial;
   r->frame_timer.start();
   r->prepare_timer.start();
   r->sample_count = 0;

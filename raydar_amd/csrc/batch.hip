@@ -843,6 +843,7 @@ class SoloLane {
   std::string err_;
   HbmPool *pool_ = nullptr;
   DevJob *d_job_ = nullptr;
+  uint64_t frame_serial_ = 0;  // one per solo job (Engine::hit_frame)
   bool inited_ = false;
   uint64_t accounted_ = 0;  // engine bytes already counted in the pool
 
@@ -895,6 +896,7 @@ class SoloLane {
                      "upload job", &err_);
     }
     const std::vector<DevJob> host{h};
+    ++frame_serial_;  // the job's passes share its camera-ray hits
     for (uint32_t done = 0; ok && done < j.spp;) {
       // passes as large as the radiance arena allows, in sample order
       DevTicket t{};
@@ -902,6 +904,7 @@ class SoloLane {
       t.sample_base = done;
       t.spp = (uint32_t)std::min<uint64_t>(j.spp - done, fit);
       t.paths = (uint32_t)(t.spp * npix);
+      eng.hit_frame = frame_serial_;
       ok = eng.run_tickets(d_job_, {t}, host, &err_);
       done += t.spp;
     }

@@ -84,6 +84,7 @@ void Engine::release_buffers() {
   if (L) { (void)hipFree(L); L = nullptr; }
   if (first_hit_) { (void)hipFree(first_hit_); first_hit_ = nullptr; }
   first_hit_px_ = 0;
+  first_hit_frame_ = 0;
   q_alloc = 0;
   l_alloc = 0;
   if (d_tickets) { (void)hipFree(d_tickets); d_tickets = nullptr; }
@@ -415,6 +416,8 @@ bool Engine::launch_iteration(const IterArgs &a, int parity, std::string *err, u
 
 bool Engine::run_tickets(const DevJob *d_jobs, std::vector<DevTicket> tickets,
                          const std::vector<DevJob> &host_jobs, std::string *err) {
+  const uint64_t frame = hit_frame;  // set by the caller for this pass only
+  hit_frame = 0;
   if (tickets.empty()) return true;
   if (!ensure_buffers(err)) return false;
   uint64_t w = 0;
@@ -594,7 +597,7 @@ bool Engine::run_tickets(const DevJob *d_jobs, std::vector<DevTicket> tickets,
     // hit; from the second iteration on, fresh paths take their pixel's hit
     // and extend traces only the survivors (same hits, same bits: the camera
     // ray of a pixel does not depend on the sample).
-    bool first_hits = !external_ && !opt.vk && host_jobs.size() == 1 && pipes[0].a.single_job &&
+    bool first_hits = !external_ && host_jobs.size() == 1 && pipes[0].a.single_job &&
                       !opt.brute_force;
     uint64_t npix_fh = first_hits ? (uint64_t)host_jobs[0].width * host_jobs[0].height : 0;
     for (const Pipe &p : pipes) first_hits = first_hits && p.e->q_alloc >= npix_fh && p.w >= npix_fh;
@@ -602,20 +605,25 @@ bool Engine::run_tickets(const DevJob *d_jobs, std::vector<DevTicket> tickets,
       if (first_hit_) (void)hipFree(first_hit_);
       first_hit_ = nullptr;
       first_hit_px_ = 0;
+      first_hit_frame_ = 0;
       if (!check(hipMalloc(&first_hit_, sizeof(HitWord) * npix_fh), "hipMalloc(first hits)", err))
         return false;
       first_hit_px_ = npix_fh;
     }
+    // an earlier pass of the same frame recorded them: every iteration reuses
+    const bool carried = first_hits && frame != 0 && frame == first_hit_frame_;
+    if (!carried) first_hit_frame_ = 0;  // rewritten by this pass's first iterations
     for (Pipe &p : pipes) p.a.first_hit = first_hits ? first_hit_ : nullptr;
     const int group = std::max(1, opt.sync_every);
     for (;;) {
       for (int g = 0; g < group; ++g)
         for (Pipe &p : pipes) {
           if (p.done) continue;
-          p.a.first_hit_mode = first_hits ? (p.it == 0 ? 1 : 2) : 0;
+          const bool record = first_hits && !carried && p.it == 0;
+          p.a.first_hit_mode = first_hits ? (record ? 1 : 2) : 0;
           // the recording iteration generates one sample only: the pipe's
           // other camera paths all take their hits from it
-          p.a.capacity = first_hits && p.it == 0 ? (uint32_t)npix_fh : p.e->q_alloc;
+          p.a.capacity = record ? (uint32_t)npix_fh : p.e->q_alloc;
           if (!p.e->launch_iteration(p.a, p.parity, err, p.bound)) return false;
           p.parity ^= 1;
           ++p.it;
@@ -654,6 +662,9 @@ bool Engine::run_tickets(const DevJob *d_jobs, std::vector<DevTicket> tickets,
       if (all) break;
     }
     drain.ok = true;
+    // the table now holds this frame's hits (every pipe ran its recording
+    // iteration, or they were already there)
+    if (first_hits) first_hit_frame_ = frame;
     for (int k = 1; k < np; ++k) {  // the pipes' kernel statistics are this engine's
       Engine &e = *aux_[k - 1];
       for (int i = 0; i < 5; ++i) {

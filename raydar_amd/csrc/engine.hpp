@@ -150,6 +150,10 @@ class Engine {
   uint64_t kernel_launches[5] = {0, 0, 0, 0, 0};
   uint64_t iterations = 0;
   int pipes_used = 0;  // pipes of the last traced pass
+  // Frame of the next run_tickets pass (0: none): passes of one frame whose
+  // scene and camera are unchanged reuse the camera-ray hits the frame's
+  // first pass recorded instead of recording them again.
+  uint64_t hit_frame = 0;
 
   // buffers (public so the server can drive iterations itself)
   PathQueue q[2] = {};
@@ -193,6 +197,7 @@ class Engine {
   uint32_t tickets_pin_cap_ = 0;
   HitWord *first_hit_ = nullptr;              // per-pixel camera-ray hits (IterArgs::first_hit)
   uint64_t first_hit_px_ = 0;                 // pixels first_hit_ holds
+  uint64_t first_hit_frame_ = 0;              // hit_frame whose hits first_hit_ holds (0: none)
 };
 
 // Message returned by rd_last_error(NULL) (calls with no renderer handle).

@@ -396,6 +396,35 @@ def test_camera_ray_hits_reused_after_each_pipes_first_sample(opts, reuse_pipes)
     np.testing.assert_array_equal(acc.view(np.uint32), ref.view(np.uint32))
 
 
+def test_camera_ray_hits_carry_across_a_frames_passes():
+    # progressive rendering, one sample per call: the frame's first pass
+    # records every pixel's camera-ray hit and its later passes reuse them;
+    # restart_frame and new_frame start a frame that records them again
+    sc = scenegen.plain_1080p(1, 96, 54)
+    npix, spp = sc.width * sc.height, 5
+    with ra.HipRenderer(max_sample_count=spp, max_bounces=12, seed=4) as r:
+        r.reset_counters()
+        r.set_option("count", 1)
+        r.new_frame(sc)
+        for _ in range(spp):
+            r.render_sample(sc)
+        c = r.counters()
+        acc = r.accumulator()
+        assert c["reused_hits"] == (spp - 1) * npix
+        ref = oracle.render(sc, spp, 12, 4)
+        np.testing.assert_array_equal(acc.view(np.uint32), ref.view(np.uint32))
+        for start in (r.restart_frame, lambda: r.new_frame(scenegen.plain_1080p(2, 96, 54))):
+            start()
+            r.reset_counters()
+            r.render_sample()
+            assert r.counters()["reused_hits"] == 0
+            r.render_sample()
+            assert r.counters()["reused_hits"] == npix
+        acc = r.accumulator()
+    ref = oracle.render(scenegen.plain_1080p(2, 96, 54), 2, 12, 4)
+    np.testing.assert_array_equal(acc.view(np.uint32), ref.view(np.uint32))
+
+
 def test_event_counts_agree_between_trees():
     # the counting pass behind the roofline figure: the two trees give the
     # same paths, so every per-path event agrees and only the traversal
