@@ -238,7 +238,8 @@ rd_status rd_read_accumulator(rd_renderer *r, float *out, size_t float_capacity)
  *                 (1..4; 1 = one stream; 0 = automatic, the default: 2 for
  *                 scenes traversed from LDS, else 3); read-only
  *                 "pipes_used" reports the last pass's count
- *   "shade_blocks" shade workgroups per CU (0 = as many as fit)
+ *   "shade_blocks" shade workgroups per CU (0 = 7 waves per SIMD for one
+ *                  scene, as many as fit for several)
  *   "count"       1 = collect algorithmic event counters in the next passes
  *   "semantics"   0 = the CPU renderer's (default); 1 = the Vulkan renderer's
  *                 (src/renderer/vulkan.rs, shaders/vulkan/raytrace.rgen):
