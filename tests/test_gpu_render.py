@@ -373,13 +373,16 @@ def test_wide_stack_scene_matches_oracle():
     np.testing.assert_array_equal(job.acc.view(np.uint32), ref.view(np.uint32))
 
 
-@pytest.mark.parametrize("opts,reuse_pipes", [(dict(), 1), (dict(pipes=2, queue=16384), 2),
-                                              (dict(queue=4096), None),
-                                              (dict(lds_bvh=0, bvh4=0), 1)])
+@pytest.mark.parametrize("opts,reuse_pipes", [(dict(pipes=1, queue=16384), 1),
+                                              (dict(pipes=2, queue=16384), 2),
+                                              (dict(lds_bvh=0, bvh4=0, pipes=1, queue=16384), 1),
+                                              (dict(queue=4096), None), (dict(), None)])
 def test_camera_ray_hits_reused_after_each_pipes_first_sample(opts, reuse_pipes):
-    # a pixel's camera ray does not depend on the sample (cpu.rs:234-251): each
-    # pipe traces one sample's camera rays and gives its later camera paths
-    # their pixel's recorded hit; a queue smaller than one sample traces all
+    # a pixel's camera ray does not depend on the sample (cpu.rs:234-251): a
+    # pipe whose pass needs several fills of its queue traces one sample's
+    # camera rays and gives its later camera paths their pixel's recorded
+    # hit; a queue smaller than one sample, or a pass that fits in one fill
+    # (41472 paths in a 16 Mi queue), traces them all
     sc = scenegen.plain_1080p(1, 96, 54)
     npix, spp = sc.width * sc.height, 8
     with ra.HipRenderer(max_sample_count=spp, max_bounces=12, seed=4, **opts) as r:
