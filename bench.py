@@ -335,10 +335,11 @@ def main():
     ap.add_argument("--phase-gap", type=float, default=0.0,
                     help="seconds of idle GPU before each pass of the batched comparison")
     ap.add_argument("--autotune", action="store_true",
-                    help="refused: see DESIGN.md section 6.1")
+                    help="refused: a full run with this autotune and a 30 s server once ended "
+                         "in a GPU fault whose cause has not been shown")
     ap.add_argument("--serve-seconds", type=float, default=1.5,
                     help="length of the Poisson serving run (default 1.5 s, the length the "
-                         "GPU tests run; 0: skipped; longer runs are refused, DESIGN.md 6.1)")
+                         "GPU tests run; 0: skipped; longer runs are refused, as --autotune is)")
     ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--option", action="append", default=[], help="key=value renderer option")
     ap.add_argument("--timed-only", action="store_true",
@@ -355,12 +356,13 @@ def main():
         calibrate_copy(args.calibrate_copy)
         return
     # A run of the full benchmark with a 30 s server and the 8-scene autotune
-    # once ended the GPU box, and no cause has been shown (host orchestration
-    # cleared on a fake device under sanitizers): those two cases stay off
-    # the GPU.
+    # once ended in a GPU fault, and no cause has been shown (host
+    # orchestration cleared on a fake device under sanitizers): those two
+    # cases stay off the GPU.
     if args.serve_seconds > 1.5 or args.autotune:
         print("bench.py: serving runs longer than 1.5 s and --autotune are refused until the cause "
-              "of the GPU fault recorded in DESIGN.md section 6.1 is shown", file=sys.stderr)
+              "of an earlier GPU fault (a full run with a 30 s server and the autotune) is shown",
+              file=sys.stderr)
         sys.exit(2)
 
     if args.batched_only:
