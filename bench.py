@@ -428,6 +428,25 @@ def main():
     ext_flop_per_launch = extend_flops(cnt) / max(ext_n, 1)
     ext_avg_s = ext_ms / 1e3 / max(ext_n, 1)
     achieved = ext_flop_per_launch / ext_avg_s / 1e12 if ext_avg_s > 0 else 0.0
+    # the same kernel on every camera ray (hit_reuse 0: each sample traces
+    # its pixel's camera ray, as the reference does): one counting and one
+    # kernel-timing frame, for comparison with the production figure above
+    r.set_option("hit_reuse", 0)
+    r.reset_counters()
+    r.set_option("count", 1)
+    step()
+    r.set_option("count", 0)
+    cnt_nr = r.counters()
+    r.reset_counters()
+    r.set_option("kernel_timing", 1)
+    step()
+    r.set_option("kernel_timing", 0)
+    kt_nr = r.kernel_times()
+    r.set_option("hit_reuse", 1)
+    nr_ms, nr_n = kt_nr["extend"]
+    nr_flop = extend_flops(cnt_nr) / max(nr_n, 1)
+    nr_s = nr_ms / 1e3 / max(nr_n, 1)
+    nr_achieved = nr_flop / nr_s / 1e12 if nr_s > 0 else 0.0
     rays_per_step = cnt["segments"]
     traffic, traffic_note = extend_traffic(args.workload)
     total_flop = path_flops(cnt)
@@ -476,6 +495,18 @@ def main():
             "avg_launch_ms": round(ext_avg_s * 1e3, 5),
             "launches": ext_n,
             "timing": "each launch timed alone (kernel-timing pass, launches serialized)",
+            "every_camera_ray_traced": {
+                "note": "same kernel with option hit_reuse 0 (each sample traces its pixel's "
+                        "camera ray, as the reference does; same image): the camera rays it "
+                        "then traces are its cheapest, most coherent work",
+                "achieved": round(nr_achieved, 4),
+                "frac": round(nr_achieved / PEAK_FP32_TFLOPS, 6),
+                "frac_of_issue_ceiling": round(nr_achieved / ISSUE_TOPS, 6),
+                "avg_launch_ms": round(nr_s * 1e3, 5),
+                "launches": nr_n,
+                "rays_per_step": cnt_nr["segments"],
+                "extend_ms_per_step": round(nr_ms, 3),
+            },
         },
         "kernel_ms_per_step": {k: round(v[0], 3) for k, v in kt.items()},
         "kernel_launches_per_step": {k: v[1] for k, v in kt.items()},

@@ -240,6 +240,10 @@ rd_status rd_read_accumulator(rd_renderer *r, float *out, size_t float_capacity)
  *                 "pipes_used" reports the last pass's count
  *   "shade_blocks" shade workgroups per CU (0 = 7 waves per SIMD for one
  *                  scene, as many as fit for several)
+ *   "hit_reuse"    1 (default): a pixel's camera ray, the same for every
+ *                  sample, is traced once per pass (or frame) and its hit
+ *                  reused by the other samples; 0: traced for every sample
+ *                  (same images)
  *   "count"       1 = collect algorithmic event counters in the next passes
  *   "semantics"   0 = the CPU renderer's (default); 1 = the Vulkan renderer's
  *                 (src/renderer/vulkan.rs, shaders/vulkan/raytrace.rgen):

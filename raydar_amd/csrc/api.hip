@@ -319,6 +319,7 @@ bool option_value(const rd_renderer *r, const std::string &k, int64_t *v) {
   else if (k == "lds_bvh") *v = o.lds_bvh;
   else if (k == "lds_objects") *v = o.lds_objects;
   else if (k == "shade_blocks") *v = o.shade_blocks;
+  else if (k == "hit_reuse") *v = o.hit_reuse;
   else if (k == "bvh4") *v = o.bvh4;
   else if (k == "pipes") *v = o.pipes;
   else if (k == "pipes_used") *v = r->eng.pipes_used;
@@ -691,6 +692,9 @@ rd_status rd_set_option(rd_renderer *r, const char *key, int64_t v) {
   } else if (k == "pipes") {
     if (v < 0 || v > 4) return r->fail(RD_ERR_INVALID_ARGUMENT, "pipes must be in 0..4 (0 = automatic)");
     o.pipes = (int)v;
+  } else if (k == "hit_reuse") {
+    if (v != 0 && v != 1) return r->fail(RD_ERR_INVALID_ARGUMENT, "hit_reuse must be 0 or 1");
+    o.hit_reuse = (int)v;
   } else if (k == "shade_blocks") {
     if (v < 0 || v > 64) return r->fail(RD_ERR_INVALID_ARGUMENT, "shade_blocks must be in 0..64");
     o.shade_blocks = (int)v;

@@ -603,8 +603,8 @@ bool Engine::run_tickets(const DevJob *d_jobs, std::vector<DevTicket> tickets,
     // hit; from the second iteration on, fresh paths take their pixel's hit
     // and extend traces only the survivors (same hits, same bits: the camera
     // ray of a pixel does not depend on the sample).
-    bool first_hits = !external_ && host_jobs.size() == 1 && pipes[0].a.single_job &&
-                      !opt.brute_force;
+    bool first_hits = opt.hit_reuse && !external_ && host_jobs.size() == 1 &&
+                      pipes[0].a.single_job && !opt.brute_force;
     uint64_t npix_fh = first_hits ? (uint64_t)host_jobs[0].width * host_jobs[0].height : 0;
     for (const Pipe &p : pipes) first_hits = first_hits && p.e->q_alloc >= npix_fh && p.w >= npix_fh;
     if (first_hits && npix_fh > first_hit_px_) {

@@ -87,6 +87,7 @@ struct EngineOptions {
   uint32_t lane_chunk_log2 = 6;   // closest-hit lanes: the queue's chunks of 64 paths
                                   // dealt round-robin to the waves (0: one contiguous
                                   // range per wave; lone extend launches 10-22% slower)
+  int hit_reuse = 1;              // 1: camera-ray hits traced once per pixel and reused
   int vk = 0;                     // 1: the Vulkan renderer's semantics (per-scene
                                   // renderer only)
 };

@@ -376,7 +376,8 @@ def test_wide_stack_scene_matches_oracle():
 @pytest.mark.parametrize("opts,reuse_pipes", [(dict(pipes=1, queue=16384), 1),
                                               (dict(pipes=2, queue=16384), 2),
                                               (dict(lds_bvh=0, bvh4=0, pipes=1, queue=16384), 1),
-                                              (dict(queue=4096), None), (dict(), None)])
+                                              (dict(queue=4096), None), (dict(), None),
+                                              (dict(pipes=1, queue=16384, hit_reuse=0), None)])
 def test_camera_ray_hits_reused_after_each_pipes_first_sample(opts, reuse_pipes):
     # a pixel's camera ray does not depend on the sample (cpu.rs:234-251): a
     # pipe whose pass needs several fills of its queue traces one sample's
